@@ -1,0 +1,232 @@
+/*
+ * rtgpu.h -- C ABI of the MI355X ray-tracing hot path.
+ *
+ * This is the drop-in boundary for the reference's Vulkan raytrace path.  Each
+ * entry point names the reference interface it replaces.  Reference paths are
+ * relative to RaytracerGPU_MastersProject/.
+ *
+ *   descriptor set of raytraceBVH.comp (RaytracerBVH.cpp:202-238, 677-685):
+ *     binding 0 ParameterUBO      -> rt_params (byte-identical, 80 B)
+ *     binding 1 RGBA32F image     -> float *accum (device, W*H*4, row-major)
+ *     binding 2 Triangle SSBO     -> rt_triangle[] (64 B)
+ *     binding 3 Sphere SSBO       -> rt_sphere[]   (32 B)
+ *     binding 4 Material SSBO     -> rt_material[] (32 B)
+ *     binding 5 HLBVHNode SSBO    -> rt_bvh_node[] (40 B, the RaytracerBVH builder output)
+ *
+ * Conventions: every entry point returns an rt_status (0 = RT_OK).  On failure
+ * rt_last_error(ctx) holds a message; nothing is thrown across the ABI.  A
+ * context is used from one host thread at a time.  "device" pointers are HIP
+ * device allocations on the context's GPU; "host" pointers are borrowed for the
+ * duration of the call only.  Streams are hipStream_t passed as void* (NULL =
+ * the null stream); work is stream-ordered and the call returns without
+ * waiting unless stated.
+ */
+#ifndef RTGPU_H
+#define RTGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rt_status {
+    RT_OK = 0,
+    RT_ERR_INVALID = 1,    /* bad argument, size or layout                     */
+    RT_ERR_NO_MEMORY = 2,  /* device allocation failed                         */
+    RT_ERR_HIP = 3,        /* HIP runtime error                                */
+    RT_ERR_NOT_FOUND = 4,  /* unknown scene / request id                       */
+    RT_ERR_BUDGET = 5,     /* HBM budget of a server would be exceeded         */
+    RT_ERR_OVERFLOW = 6,   /* traversal stack bound exceeded (reported, rays terminated) */
+    RT_ERR_BUSY = 7        /* request not finished yet                         */
+} rt_status;
+
+typedef struct rt_context rt_context;
+
+/* ---------------- byte layouts (see static_asserts in the implementation) ---------------- */
+
+/* definitions.glsl:15-21 / SceneTypes.hpp:41-47 (GPU::Triangle), 64 B */
+typedef struct rt_triangle {
+    float v0[4], v1[4], v2[4];
+    uint32_t materialIndex, modelIndex, _pad[2];
+} rt_triangle;
+
+/* definitions.glsl:23-28 / SceneTypes.hpp:62-66 (GPU::Sphere), 32 B */
+typedef struct rt_sphere {
+    float center[4];
+    float radius;
+    uint32_t materialIndex, modelIndex, _pad;
+} rt_sphere;
+
+/* definitions.glsl:10-13 / SceneTypes.hpp:82-84 (GPU::Material), 32 B.
+ * albedo[3] is unused by the reference; here it carries fuzz (METALLIC) or the
+ * index of refraction (DIELECTRIC). */
+typedef struct rt_material {
+    float albedo[4];
+    uint32_t materialType; /* RT_LIGHT, RT_DIFFUSE, RT_METALLIC, RT_DIELECTRIC */
+    uint32_t _pad[3];
+} rt_material;
+
+/* definitions.glsl:52-72 / SceneTypes.hpp:107-123 (GPU::BVHNode), 40 B.
+ * 2N-1 nodes; 0 is the root; internal nodes 0..N-2; leaves N-1..2N-2;
+ * a leaf has left == right == 0. */
+typedef struct rt_bvh_node {
+    float minX, maxX, minY, maxY, minZ, maxZ;
+    uint32_t left, right, primitiveIndex, primitiveType;
+} rt_bvh_node;
+
+/* raytraceBVH.comp:7-18 (std140) / RaytracerBVH.hpp:31-42, 80 B */
+typedef struct rt_params {
+    float camPos[4], camLookAt[4], camUpDir[4];
+    float verticalFOV; /* degrees */
+    uint32_t numTriangles, numSpheres, numMaterials, numLights, maxRayTraceDepth, randomState;
+    uint32_t _pad;
+} rt_params;
+
+/* ray query records: 32 B ray, 16 B hit */
+typedef struct rt_ray { float o[3]; float tmin; float d[3]; float tmax; } rt_ray;
+typedef struct rt_hit { uint32_t prim; float t; float u; float v; } rt_hit;
+
+enum { RT_LIGHT = 0, RT_DIFFUSE = 1, RT_METALLIC = 2, RT_DIELECTRIC = 3 };   /* definitions.glsl:79-82 */
+enum { RT_SPHERE_PRIMITIVE = 0, RT_TRIANGLE_PRIMITIVE = 1 };                   /* definitions.glsl:84-85 */
+#define RT_NO_HIT 0xFFFFFFFFu
+#define RT_SPHERE_BIT 0x80000000u /* rt_hit.prim of a sphere hit = RT_SPHERE_BIT | sphere index */
+
+/* ---------------- context ---------------- */
+
+/* Replaces Device{window} + Raytracer::initVulkan (RaytracerBVH.cpp:7-11, RaytracerBVH.hpp:145-178). */
+int rt_create(rt_context **out, int device);
+void rt_destroy(rt_context *ctx);
+const char *rt_last_error(const rt_context *ctx);
+
+/* ---------------- scenes ---------------- */
+
+#define RT_BUILD_REFERENCE_LEAVES 1u /* ConstructHLBVH.comp:152-169: leaf N-1+i holds primitive i (unsorted) */
+#define RT_BUILD_REFERENCE_MORTON 2u /* GenerateMortonCodesOfPrimitives.comp:60-65: coord/span quantisation */
+#define RT_BUILD_KEEP_BINARY 4u      /* keep the 2N-1 binary nodes resident for rt_scene_get_nodes */
+
+typedef struct rt_build_options {
+    uint32_t flags;     /* RT_BUILD_* */
+    uint32_t width;     /* wide-BVH width: 2, 4 or 8 (0 = 4) */
+    uint32_t leaf_size; /* max primitives per wide leaf, 1..8 (0 = 1) */
+    uint32_t _reserved;
+} rt_build_options;
+
+/* Upload a scene and build its BVH on the device.  Replaces RaytraceScene::prepForRender
+ * (RaytraceScene.cpp:73-76, 157-200) plus compute stage S1 (RaytracerBVH.cpp:734-997):
+ * model->world transform (when models != NULL; mat4 column-major, 16 floats each),
+ * enclosing AABB, Morton codes, radix sort, Karras LBVH, refit; then the binary LBVH is
+ * collapsed into the resident wide layout the kernels traverse.
+ * nodes != NULL: use this reference builder output (2N-1 rt_bvh_node) instead of building.
+ * Zero triangles and zero spheres is a valid (empty) scene. */
+int rt_scene_upload(rt_context *ctx, uint32_t *scene_id, const rt_triangle *tris, uint32_t ntri,
+                    const rt_sphere *spheres, uint32_t nsph, const rt_material *materials, uint32_t nmat,
+                    const float *models, uint32_t nmodels, const rt_bvh_node *nodes, uint32_t nnodes,
+                    const rt_build_options *opts);
+int rt_scene_release(rt_context *ctx, uint32_t scene_id);
+
+typedef struct rt_scene_info {
+    uint32_t ntri, nsph, nmat, nprim;
+    uint32_t width, leaf_size, nwide, wide_depth;
+    uint32_t stack_bound, flags;
+    uint64_t device_bytes; /* resident bytes (wide nodes + leaf records + tables) */
+    float build_ms;        /* device time of the last build */
+    float _pad;
+} rt_scene_info;
+int rt_scene_info_get(rt_context *ctx, uint32_t scene_id, rt_scene_info *info);
+
+/* Download the RaytracerBVH builder output (requires RT_BUILD_KEEP_BINARY): 2N-1 nodes. */
+int rt_scene_get_nodes(rt_context *ctx, uint32_t scene_id, rt_bvh_node *host_out, uint32_t capacity);
+/* Download the leaf-slot -> primitive table (N entries; RT_SPHERE_BIT marks spheres). */
+int rt_scene_get_slots(rt_context *ctx, uint32_t scene_id, uint32_t *host_out, uint32_t capacity);
+/* Download the wide BVH (nwide nodes of 32*width bytes; root is node 0). */
+int rt_scene_get_wide(rt_context *ctx, uint32_t scene_id, void *host_out, size_t capacity_bytes);
+
+/* ---------------- ray queries ---------------- */
+
+#define RT_TRACE_CLOSEST 0u
+#define RT_TRACE_ANY 1u        /* any-hit: terminate at the first hit found in [tmin, tmax] */
+
+typedef struct rt_launch_config {
+    uint32_t block;        /* threads per block: 64, 128, 256 (0 = tuned default)       */
+    uint32_t lds_stack;    /* LDS short-stack entries per lane: 4, 8, 16 (0 = default)  */
+    uint32_t persistent;   /* 1 = persistent work-stealing grid, 0 = one lane per ray    */
+    uint32_t refill;       /* persistent: refill a wave when >= this many lanes are idle */
+} rt_launch_config;
+
+/* Closest-hit / any-hit ray query (the hitBVH/sceneHit semantics of raytraceBVH.comp:195-274
+ * with per-ray [tmin, tmax]).  rays, hits: device.  counts (device, may be NULL): per ray
+ * uint32 {AABB tests, primitive tests}.  cfg may be NULL (defaults / autotuned). */
+int rt_trace(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n, rt_hit *hits,
+             uint32_t *counts, uint32_t flags, const rt_launch_config *cfg, void *stream);
+
+/* ---------------- path tracing (compute stage S2) ---------------- */
+
+#define RT_RENDER_REFERENCE_MATERIALS 1u /* raytraceBVH.comp:100-115: METALLIC/DIELECTRIC absorb */
+#define RT_RENDER_NO_PRIMARY_CACHE 2u    /* trace the (identical) primary segment of every sample */
+
+/* Clear an RGBA32F accumulation image to (0,0,0,1) (RaytracerBVH.cpp:750-776). */
+int rt_accum_clear(rt_context *ctx, float *accum, uint32_t npix, void *stream);
+
+/* spp samples of raytraceBVH.comp's main() per pixel, exactly as spp successive
+ * dispatches separated by image barriers (RaytracerBVH.cpp:1025-1050), including the
+ * alpha-channel RNG chain.  params is host memory (the UBO).  segments (device, may be
+ * NULL): uint64 counter incremented by the number of traced segments. */
+int rt_render(rt_context *ctx, uint32_t scene_id, const rt_params *params, uint32_t width,
+              uint32_t height, uint32_t spp, uint32_t flags, float *accum, unsigned long long *segments,
+              const rt_launch_config *cfg, void *stream);
+
+/* SingleTriangleFullScreen.frag:13-21: out = clamp(sqrt(accum.rgb / spp), 0, 1), a = 1. */
+int rt_resolve(rt_context *ctx, const float *accum, uint32_t npix, uint32_t spp, float *out_rgba,
+               void *stream);
+
+/* ---------------- batched persistent launch (many scenes, one launch) ---------------- */
+
+typedef struct rt_batch_item {
+    uint32_t scene_id;
+    uint32_t nrays;
+    const rt_ray *rays; /* device */
+    rt_hit *hits;       /* device */
+} rt_batch_item;
+
+/* All items' rays are traced by one persistent work-stealing launch.  items: host array. */
+int rt_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems, uint32_t flags,
+                   const rt_launch_config *cfg, void *stream);
+
+/* ---------------- autotuner ---------------- */
+
+typedef struct rt_tune_result {
+    rt_launch_config cfg;
+    float ms;           /* best median time */
+    float rays_per_s;
+    uint32_t tried;
+    uint32_t _pad;
+} rt_tune_result;
+
+/* Times launch configurations on (scene, rays) and makes the best the context's default
+ * for that scene.  rays/hits: device; the hits buffer is overwritten. */
+int rt_autotune(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n, rt_hit *hits,
+                uint32_t flags, rt_tune_result *out);
+
+/* ---------------- device plumbing for hosts without their own HIP runtime ---------------- */
+int rt_util_malloc(void **ptr, size_t bytes);
+int rt_util_free(void *ptr);
+int rt_util_memcpy_htod(void *dst, const void *src, size_t bytes);
+int rt_util_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int rt_util_memset(void *ptr, int value, size_t bytes);
+int rt_util_synchronize(void);
+int rt_util_stream_create(void **stream);
+int rt_util_stream_destroy(void *stream);
+int rt_util_stream_synchronize(void *stream);
+int rt_util_event_create(void **event);
+int rt_util_event_destroy(void *event);
+int rt_util_event_record(void *event, void *stream);
+int rt_util_event_elapsed_ms(void *start, void *end, float *ms);
+int rt_util_device_count(int *count);
+int rt_util_device_name(int device, char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,306 @@
+// api.hip -- the extern "C" boundary (include/rtgpu.h).  Exceptions never cross it.
+#include <algorithm>
+#include <cstddef>
+#include <cstring>
+#include <vector>
+
+#include "rt_internal.hpp"
+
+using namespace rtgpu;
+
+// byte layouts of the reference's SSBOs / UBO (RaytracerGPU_MastersProject/shaders/include/definitions.glsl,
+// VulkanWrapper/SceneTypes.hpp, RaytracerBVH.hpp:31-42)
+static_assert(sizeof(rt_triangle) == 64 && offsetof(rt_triangle, v1) == 16 && offsetof(rt_triangle, v2) == 32 &&
+                  offsetof(rt_triangle, materialIndex) == 48 && offsetof(rt_triangle, modelIndex) == 52,
+              "Triangle layout");
+static_assert(sizeof(rt_sphere) == 32 && offsetof(rt_sphere, radius) == 16 && offsetof(rt_sphere, materialIndex) == 20 &&
+                  offsetof(rt_sphere, modelIndex) == 24,
+              "Sphere layout");
+static_assert(sizeof(rt_material) == 32 && offsetof(rt_material, materialType) == 16, "Material layout");
+static_assert(sizeof(rt_bvh_node) == 40 && offsetof(rt_bvh_node, left) == 24 && offsetof(rt_bvh_node, right) == 28 &&
+                  offsetof(rt_bvh_node, primitiveIndex) == 32 && offsetof(rt_bvh_node, primitiveType) == 36,
+              "HLBVHNode layout");
+static_assert(sizeof(rt_params) == 80 && offsetof(rt_params, verticalFOV) == 48 &&
+                  offsetof(rt_params, numTriangles) == 52 && offsetof(rt_params, numLights) == 64 &&
+                  offsetof(rt_params, maxRayTraceDepth) == 68 && offsetof(rt_params, randomState) == 72,
+              "ParameterUBO layout");
+static_assert(sizeof(rt_ray) == 32 && sizeof(rt_hit) == 16, "ray/hit records");
+static_assert(sizeof(LeafRecord) == 64, "leaf record");
+
+namespace {
+
+template <class F> int guarded(rt_context *ctx, F &&f) {
+    try {
+        if (ctx) {
+            hipError_t e = hipSetDevice(ctx->device);
+            if (e != hipSuccess) throw Error(RT_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+        }
+        f();
+        return RT_OK;
+    } catch (const Error &e) {
+        if (ctx) ctx->last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return RT_ERR_NO_MEMORY;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->last_error = e.what();
+        return RT_ERR_INVALID;
+    }
+}
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return RT_OK;
+    return e == hipErrorOutOfMemory ? RT_ERR_NO_MEMORY : RT_ERR_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rt_create(rt_context **out, int device) {
+    if (!out) return RT_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return RT_ERR_HIP;
+    rt_context *ctx = nullptr;
+    try {
+        ctx = new rt_context();
+    } catch (...) {
+        return RT_ERR_NO_MEMORY;
+    }
+    ctx->device = device;
+    int st = guarded(ctx, [&] {
+        RT_HIP(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device));
+        ctx->counters.alloc(16ull * rt_context::kCounterSlots);
+        RT_HIP(hipMemset(ctx->counters.p, 0, ctx->counters.bytes));
+    });
+    if (st != RT_OK) {
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return RT_OK;
+}
+
+void rt_destroy(rt_context *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    delete ctx;
+}
+
+const char *rt_last_error(const rt_context *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int rt_scene_upload(rt_context *ctx, uint32_t *scene_id, const rt_triangle *tris, uint32_t ntri,
+                    const rt_sphere *spheres, uint32_t nsph, const rt_material *materials, uint32_t nmat,
+                    const float *models, uint32_t nmodels, const rt_bvh_node *nodes, uint32_t nnodes,
+                    const rt_build_options *opts) {
+    if (!ctx || !scene_id) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        rt_build_options o{};
+        if (opts) o = *opts;
+        auto sc = std::make_unique<Scene>();
+        build_scene(ctx, *sc, tris, ntri, spheres, nsph, materials, nmat, models, nmodels, nodes, nnodes, o);
+        uint32_t id = ctx->next_scene++;
+        ctx->scenes[id] = std::move(sc);
+        *scene_id = id;
+    });
+}
+
+int rt_scene_release(rt_context *ctx, uint32_t scene_id) {
+    if (!ctx) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        auto it = ctx->scenes.find(scene_id);
+        if (it == ctx->scenes.end()) throw Error(RT_ERR_NOT_FOUND, "unknown scene id " + std::to_string(scene_id));
+        RT_HIP(hipDeviceSynchronize());  // in-flight work may still read the scene
+        ctx->scenes.erase(it);
+    });
+}
+
+int rt_scene_info_get(rt_context *ctx, uint32_t scene_id, rt_scene_info *info) {
+    if (!ctx || !info) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        const Scene &s = ctx->scene(scene_id);
+        std::memset(info, 0, sizeof(*info));
+        info->ntri = s.ntri; info->nsph = s.nsph; info->nmat = s.nmat; info->nprim = s.nprim;
+        info->width = s.width; info->leaf_size = s.leaf_size; info->nwide = s.nwide; info->wide_depth = s.wide_depth;
+        info->stack_bound = s.stack_bound; info->flags = s.flags;
+        info->device_bytes = s.device_bytes();
+        info->build_ms = s.build_ms;
+    });
+}
+
+int rt_scene_get_nodes(rt_context *ctx, uint32_t scene_id, rt_bvh_node *host_out, uint32_t capacity) {
+    if (!ctx || !host_out) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        const Scene &s = ctx->scene(scene_id);
+        RT_REQUIRE(s.binary.p != nullptr || s.nprim == 0, "scene was built without RT_BUILD_KEEP_BINARY");
+        uint64_t n = s.nprim ? 2ull * s.nprim - 1 : 0;
+        RT_REQUIRE(capacity >= n, "capacity too small");
+        if (n) RT_HIP(hipMemcpy(host_out, s.binary.p, sizeof(rt_bvh_node) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int rt_scene_get_slots(rt_context *ctx, uint32_t scene_id, uint32_t *host_out, uint32_t capacity) {
+    if (!ctx || !host_out) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        const Scene &s = ctx->scene(scene_id);
+        RT_REQUIRE(capacity >= s.nprim, "capacity too small");
+        if (s.nprim) RT_HIP(hipMemcpy(host_out, s.slot_prim.p, sizeof(uint32_t) * s.nprim, hipMemcpyDeviceToHost));
+    });
+}
+
+int rt_scene_get_wide(rt_context *ctx, uint32_t scene_id, void *host_out, size_t capacity_bytes) {
+    if (!ctx || !host_out) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        const Scene &s = ctx->scene(scene_id);
+        RT_REQUIRE(capacity_bytes >= s.wide.bytes, "capacity too small");
+        if (s.wide.bytes) RT_HIP(hipMemcpy(host_out, s.wide.p, s.wide.bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int rt_trace(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n, rt_hit *hits, uint32_t *counts,
+             uint32_t flags, const rt_launch_config *cfg, void *stream) {
+    if (!ctx) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        RT_REQUIRE((flags & ~RT_TRACE_ANY) == 0, "unknown trace flags");
+        const Scene &s = ctx->scene(scene_id);
+        LaunchCfg c = resolve_cfg(ctx, s, cfg, (flags & RT_TRACE_ANY) != 0);
+        launch_trace(ctx, s, rays, n, hits, counts, flags, c, static_cast<hipStream_t>(stream));
+    });
+}
+
+int rt_accum_clear(rt_context *ctx, float *accum, uint32_t npix, void *stream) {
+    if (!ctx) return RT_ERR_INVALID;
+    return guarded(ctx, [&] { launch_accum_clear(accum, npix, static_cast<hipStream_t>(stream)); });
+}
+
+int rt_render(rt_context *ctx, uint32_t scene_id, const rt_params *params, uint32_t width, uint32_t height,
+              uint32_t spp, uint32_t flags, float *accum, unsigned long long *segments, const rt_launch_config *cfg,
+              void *stream) {
+    if (!ctx || !params) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        RT_REQUIRE((flags & ~(RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE)) == 0,
+                   "unknown render flags");
+        const Scene &s = ctx->scene(scene_id);
+        LaunchCfg c = resolve_cfg(ctx, s, cfg, false);
+        launch_render(ctx, s, *params, width, height, spp, flags, accum, segments, c,
+                      static_cast<hipStream_t>(stream));
+    });
+}
+
+int rt_resolve(rt_context *ctx, const float *accum, uint32_t npix, uint32_t spp, float *out_rgba, void *stream) {
+    if (!ctx) return RT_ERR_INVALID;
+    return guarded(ctx, [&] { launch_resolve(accum, npix, spp, out_rgba, static_cast<hipStream_t>(stream)); });
+}
+
+int rt_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems, uint32_t flags,
+                   const rt_launch_config *cfg, void *stream) {
+    if (!ctx) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        RT_REQUIRE((flags & ~RT_TRACE_ANY) == 0, "unknown trace flags");
+        LaunchCfg c;
+        if (nitems) c = resolve_cfg(ctx, ctx->scene(items[0].scene_id), cfg, (flags & RT_TRACE_ANY) != 0);
+        launch_trace_batch(ctx, items, nitems, flags, c, static_cast<hipStream_t>(stream));
+    });
+}
+
+int rt_autotune(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n, rt_hit *hits, uint32_t flags,
+                rt_tune_result *out) {
+    if (!ctx) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        RT_REQUIRE(n > 0, "autotune needs rays");
+        Scene &s = ctx->scene(scene_id);
+        const bool any = (flags & RT_TRACE_ANY) != 0;
+        std::vector<LaunchCfg> cands;
+        for (uint32_t block : {64u, 128u, 256u})
+            for (uint32_t st : {4u, 8u, 16u}) {
+                if (s.stack_bound > st + 184u) continue;
+                cands.push_back(LaunchCfg{block, st, 0u, 16u});
+                for (uint32_t refill : {8u, 16u, 32u}) cands.push_back(LaunchCfg{block, st, 1u, refill});
+            }
+        RT_REQUIRE(!cands.empty(), "no launch configuration fits the tree's stack bound");
+        hipStream_t stream;
+        RT_HIP(hipStreamCreate(&stream));
+        hipEvent_t e0, e1;
+        RT_HIP(hipEventCreate(&e0));
+        RT_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        LaunchCfg bestc = cands[0];
+        for (const LaunchCfg &c : cands) {
+            launch_trace(ctx, s, rays, n, hits, nullptr, flags, c, stream);  // warm-up
+            float times[3];
+            for (float &tm : times) {
+                RT_HIP(hipEventRecord(e0, stream));
+                launch_trace(ctx, s, rays, n, hits, nullptr, flags, c, stream);
+                RT_HIP(hipEventRecord(e1, stream));
+                RT_HIP(hipEventSynchronize(e1));
+                RT_HIP(hipEventElapsedTime(&tm, e0, e1));
+            }
+            std::sort(times, times + 3);
+            if (times[1] < best) {
+                best = times[1];
+                bestc = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(stream);
+        if (!s.has_tuned) {
+            s.tuned_closest = LaunchCfg{};
+            s.tuned_any = LaunchCfg{};
+        }
+        (any ? s.tuned_any : s.tuned_closest) = bestc;
+        s.has_tuned = true;
+        if (out) {
+            out->cfg = rt_launch_config{bestc.block, bestc.lds_stack, bestc.persistent, bestc.refill};
+            out->ms = best;
+            out->rays_per_s = best > 0.0f ? (float)(n / (best * 1e-3)) : 0.0f;
+            out->tried = (uint32_t)cands.size();
+        }
+    });
+}
+
+// ---------------- plumbing ----------------
+int rt_util_malloc(void **ptr, size_t bytes) { return ptr ? hip_status(hipMalloc(ptr, bytes)) : RT_ERR_INVALID; }
+int rt_util_free(void *ptr) { return hip_status(hipFree(ptr)); }
+int rt_util_memcpy_htod(void *dst, const void *src, size_t bytes) {
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+int rt_util_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+int rt_util_memset(void *ptr, int value, size_t bytes) { return hip_status(hipMemset(ptr, value, bytes)); }
+int rt_util_synchronize(void) { return hip_status(hipDeviceSynchronize()); }
+int rt_util_stream_create(void **stream) {
+    return stream ? hip_status(hipStreamCreate(reinterpret_cast<hipStream_t *>(stream))) : RT_ERR_INVALID;
+}
+int rt_util_stream_destroy(void *stream) { return hip_status(hipStreamDestroy(static_cast<hipStream_t>(stream))); }
+int rt_util_stream_synchronize(void *stream) {
+    return hip_status(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+}
+int rt_util_event_create(void **event) {
+    return event ? hip_status(hipEventCreate(reinterpret_cast<hipEvent_t *>(event))) : RT_ERR_INVALID;
+}
+int rt_util_event_destroy(void *event) { return hip_status(hipEventDestroy(static_cast<hipEvent_t>(event))); }
+int rt_util_event_record(void *event, void *stream) {
+    return hip_status(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+}
+int rt_util_event_elapsed_ms(void *start, void *end, float *ms) {
+    if (!ms) return RT_ERR_INVALID;
+    hipError_t e = hipEventSynchronize(static_cast<hipEvent_t>(end));
+    if (e != hipSuccess) return hip_status(e);
+    return hip_status(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(end)));
+}
+int rt_util_device_count(int *count) { return count ? hip_status(hipGetDeviceCount(count)) : RT_ERR_INVALID; }
+int rt_util_device_name(int device, char *buf, size_t len) {
+    if (!buf || len == 0) return RT_ERR_INVALID;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return hip_status(e);
+    std::snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+    return RT_OK;
+}
+
+}  // extern "C"

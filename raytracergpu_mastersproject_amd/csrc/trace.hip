@@ -1,0 +1,311 @@
+// trace.hip -- ray-query kernels (closest-hit / any-hit) over the resident wide BVH.
+//
+// Two schedules:
+//   persistent: a grid sized to fill every CU once; each wave pulls rays from a
+//     global work counter.  When at least `refill` lanes of a wave are idle the
+//     wave takes that many new rays with one atomic (ballot + popcount + lane
+//     prefix), so finished lanes are refilled while long rays keep traversing.
+//   direct: one lane per ray (the plain baseline).
+// The batched variant runs many scenes' ray batches in one persistent launch.
+#include <algorithm>
+#include <vector>
+
+#include "rt_internal.hpp"
+#include "traverse.hpp"
+
+namespace rtgpu {
+namespace {
+
+struct SceneView {
+    const uint32_t *wide;
+    const LeafRecord *rec;
+    const uint32_t *slot_prim;
+    uint32_t nwide;
+    uint32_t _pad;
+};
+
+struct TraceArgs {
+    SceneView sc;
+    const rt_ray *rays;
+    rt_hit *hits;
+    uint32_t *counts;
+    uint32_t n;
+    uint32_t refill;
+    unsigned int *work;
+};
+
+__device__ __forceinline__ void load_ray(const rt_ray *rays, uint32_t i, v3 &o, float &tmin, v3 &d, float &tmax) {
+    const float4 *p = reinterpret_cast<const float4 *>(rays + i);
+    float4 a = p[0], b = p[1];
+    o = v3{a.x, a.y, a.z};
+    tmin = a.w;
+    d = v3{b.x, b.y, b.z};
+    tmax = b.w;
+}
+
+__device__ __forceinline__ void store_hit(const SceneView &sc, const Trav &t, rt_hit *hits, uint32_t *counts,
+                                          uint32_t i) {
+    uint4 h;
+    if (t.bslot == kEmpty) h = make_uint4(RT_NO_HIT, __float_as_uint(t.bt), 0u, 0u);
+    else h = make_uint4(sc.slot_prim[t.bslot], __float_as_uint(t.bt), __float_as_uint(t.bu), __float_as_uint(t.bv));
+    reinterpret_cast<uint4 *>(hits)[i] = h;
+    if (counts) reinterpret_cast<uint2 *>(counts)[i] = make_uint2(t.nbox, t.ntri);
+}
+
+template <int W, int S, bool ANY>
+__global__ __launch_bounds__(256) void k_trace_direct(TraceArgs a) {
+    extern __shared__ uint2 lds[];
+    uint2 spill[kSpillCap];
+    StackMem m{lds + threadIdx.x, blockDim.x, spill};
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    v3 o, d;
+    float tmin, tmax;
+    load_ray(a.rays, i, o, tmin, d, tmax);
+    Trav t;
+    trav_init(t, o, d, tmin, tmax, a.sc.nwide);
+    while (!trav_done(t)) trav_step<W, S, ANY>(t, m, a.sc.wide, a.sc.rec);
+    store_hit(a.sc, t, a.hits, a.counts, i);
+}
+
+template <int W, int S, bool ANY>
+__global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
+    extern __shared__ uint2 lds[];
+    uint2 spill[kSpillCap];
+    StackMem m{lds + threadIdx.x, blockDim.x, spill};
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
+    int64_t rid = -1;
+    bool exhausted = false;
+    Trav t;
+    t.have = false;
+    t.top = 0;
+    for (;;) {
+        const unsigned long long idle = __ballot(rid < 0);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && nidle >= a.refill) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(a.work, nidle);
+            base = (uint32_t)__shfl((int)base, leader);
+            if ((uint64_t)base + nidle >= a.n) exhausted = true;
+            if (rid < 0) {
+                const uint64_t mine = (uint64_t)base + (uint64_t)__popcll(idle & lt);
+                if (mine < a.n) {
+                    rid = (int64_t)mine;
+                    v3 o, d;
+                    float tmin, tmax;
+                    load_ray(a.rays, (uint32_t)mine, o, tmin, d, tmax);
+                    trav_init(t, o, d, tmin, tmax, a.sc.nwide);
+                }
+            }
+        }
+        if (__ballot(rid >= 0) == 0ull) break;  // every lane idle and the queue is drained
+        if (rid >= 0) {
+            trav_step<W, S, ANY>(t, m, a.sc.wide, a.sc.rec);
+            if (trav_done(t)) {
+                store_hit(a.sc, t, a.hits, a.counts, (uint32_t)rid);
+                rid = -1;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched: many (scene, rays) items in one persistent launch.  A global ray
+// index runs over the concatenation of all items; item boundaries are found by
+// binary search in an exclusive prefix of item sizes held in the argument block.
+// ---------------------------------------------------------------------------
+struct BatchItemDev {
+    SceneView sc;
+    const rt_ray *rays;
+    rt_hit *hits;
+    uint64_t first;  // global index of the item's first ray
+};
+
+struct BatchArgs {
+    const BatchItemDev *items;
+    uint32_t nitems;
+    uint32_t refill;
+    uint64_t total;
+    unsigned long long *work;
+};
+
+__device__ __forceinline__ uint32_t find_item(const BatchItemDev *items, uint32_t nitems, uint64_t g) {
+    uint32_t lo = 0, hi = nitems;  // last item with first <= g
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (items[mid].first <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int W, int S, bool ANY>
+__global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
+    extern __shared__ uint2 lds[];
+    uint2 spill[kSpillCap];
+    StackMem m{lds + threadIdx.x, blockDim.x, spill};
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
+    bool active = false, exhausted = false;
+    uint32_t item = 0, local = 0;
+    Trav t;
+    t.have = false;
+    t.top = 0;
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && nidle >= a.refill) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            unsigned long long base = 0;
+            if ((int)lane == leader) base = atomicAdd(a.work, (unsigned long long)nidle);
+            base = __shfl(base, leader);
+            if (base + nidle >= a.total) exhausted = true;
+            if (!active) {
+                const uint64_t g = base + (uint64_t)__popcll(idle & lt);
+                if (g < a.total) {
+                    item = find_item(a.items, a.nitems, g);
+                    local = (uint32_t)(g - a.items[item].first);
+                    v3 o, d;
+                    float tmin, tmax;
+                    load_ray(a.items[item].rays, local, o, tmin, d, tmax);
+                    trav_init(t, o, d, tmin, tmax, a.items[item].sc.nwide);
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active) {
+            const BatchItemDev &it = a.items[item];
+            trav_step<W, S, ANY>(t, m, it.sc.wide, it.sc.rec);
+            if (trav_done(t)) {
+                store_hit(it.sc, t, it.hits, nullptr, local);
+                active = false;
+            }
+        }
+    }
+}
+
+__global__ void k_zero_counter(unsigned int *c) {
+    if (threadIdx.x == 0) { c[0] = 0u; c[1] = 0u; }
+}
+
+SceneView view_of(const Scene &sc) {
+    return SceneView{sc.wide.as<uint32_t>(), sc.records.as<LeafRecord>(), sc.slot_prim.as<uint32_t>(), sc.nwide, 0u};
+}
+
+template <int W, int S> struct Kernels {
+    static void *direct(bool any) {
+        return any ? (void *)k_trace_direct<W, S, true> : (void *)k_trace_direct<W, S, false>;
+    }
+    static void *persistent(bool any) {
+        return any ? (void *)k_trace_persistent<W, S, true> : (void *)k_trace_persistent<W, S, false>;
+    }
+    static void *batch(bool any) {
+        return any ? (void *)k_trace_batch<W, S, true> : (void *)k_trace_batch<W, S, false>;
+    }
+};
+
+template <int W> void *pick(uint32_t s, int which, bool any) {
+    switch (s) {
+        case 4: return which == 0 ? Kernels<W, 4>::direct(any) : which == 1 ? Kernels<W, 4>::persistent(any) : Kernels<W, 4>::batch(any);
+        case 16: return which == 0 ? Kernels<W, 16>::direct(any) : which == 1 ? Kernels<W, 16>::persistent(any) : Kernels<W, 16>::batch(any);
+        default: return which == 0 ? Kernels<W, 8>::direct(any) : which == 1 ? Kernels<W, 8>::persistent(any) : Kernels<W, 8>::batch(any);
+    }
+}
+
+void *pick_kernel(uint32_t width, uint32_t s, int which, bool any) {
+    switch (width) {
+        case 2: return pick<2>(s, which, any);
+        case 8: return pick<8>(s, which, any);
+        default: return pick<4>(s, which, any);
+    }
+}
+
+uint32_t persistent_grid(rt_context *ctx, const void *kernel, uint32_t block, size_t lds) {
+    int per_cu = 0;
+    RT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, lds));
+    return (uint32_t)std::max(1, per_cu) * (uint32_t)ctx->num_cus;
+}
+
+void check_stack(const Scene &sc, const LaunchCfg &c) {
+    if (sc.stack_bound > c.lds_stack + kSpillCap)
+        throw Error(RT_ERR_OVERFLOW, "traversal stack bound " + std::to_string(sc.stack_bound) +
+                                         " exceeds the kernel's capacity " + std::to_string(c.lds_stack + kSpillCap));
+}
+
+}  // namespace
+
+LaunchCfg resolve_cfg(const rt_context *ctx, const Scene &sc, const rt_launch_config *cfg, bool any) {
+    (void)ctx;
+    LaunchCfg c = sc.has_tuned ? (any ? sc.tuned_any : sc.tuned_closest) : LaunchCfg{};
+    if (cfg) {
+        if (cfg->block) c.block = cfg->block;
+        if (cfg->lds_stack) c.lds_stack = cfg->lds_stack;
+        c.persistent = cfg->persistent;
+        if (cfg->refill) c.refill = cfg->refill;
+    }
+    RT_REQUIRE(c.block == 64 || c.block == 128 || c.block == 256, "block must be 64, 128 or 256");
+    RT_REQUIRE(c.lds_stack == 4 || c.lds_stack == 8 || c.lds_stack == 16, "lds_stack must be 4, 8 or 16");
+    c.refill = std::min<uint32_t>(std::max<uint32_t>(c.refill, 1u), 64u);
+    return c;
+}
+
+void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t n, rt_hit *hits, uint32_t *counts,
+                  uint32_t flags, const LaunchCfg &c, hipStream_t stream) {
+    if (n == 0) return;
+    RT_REQUIRE(rays && hits, "rays/hits must be device pointers");
+    check_stack(sc, c);
+    const bool any = (flags & RT_TRACE_ANY) != 0;
+    TraceArgs a{view_of(sc), rays, hits, counts, n, c.refill, nullptr};
+    const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
+    if (!c.persistent) {
+        void *k = pick_kernel(sc.width, c.lds_stack, 0, any);
+        uint32_t grid = (n + c.block - 1) / c.block;
+        void *args[] = {&a};
+        RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
+    } else {
+        a.work = ctx->counter_slot();
+        k_zero_counter<<<1, 64, 0, stream>>>(a.work);
+        void *k = pick_kernel(sc.width, c.lds_stack, 1, any);
+        uint32_t grid = persistent_grid(ctx, k, c.block, lds);
+        void *args[] = {&a};
+        RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
+    }
+    RT_HIP(hipGetLastError());
+}
+
+void launch_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems, uint32_t flags,
+                        const LaunchCfg &c, hipStream_t stream) {
+    if (nitems == 0) return;
+    RT_REQUIRE(items, "items is NULL");
+    std::vector<BatchItemDev> hv(nitems);
+    uint64_t total = 0;
+    uint32_t width = 0;
+    for (uint32_t i = 0; i < nitems; ++i) {
+        const Scene &sc = ctx->scene(items[i].scene_id);
+        if (i == 0) width = sc.width;
+        RT_REQUIRE(sc.width == width, "rt_trace_batch: all scenes must share one BVH width");
+        RT_REQUIRE(items[i].nrays == 0 || (items[i].rays && items[i].hits), "rt_trace_batch: NULL rays/hits");
+        check_stack(sc, c);
+        hv[i] = BatchItemDev{view_of(sc), items[i].rays, items[i].hits, total};
+        total += items[i].nrays;
+    }
+    if (total == 0) return;
+    DevBuf ditems(sizeof(BatchItemDev) * nitems);
+    RT_HIP(hipMemcpyAsync(ditems.p, hv.data(), sizeof(BatchItemDev) * nitems, hipMemcpyHostToDevice, stream));
+    unsigned long long *work = reinterpret_cast<unsigned long long *>(ctx->counter_slot());
+    k_zero_counter<<<1, 64, 0, stream>>>(reinterpret_cast<unsigned int *>(work));
+    BatchArgs a{ditems.as<BatchItemDev>(), nitems, c.refill, total, work};
+    const bool any = (flags & RT_TRACE_ANY) != 0;
+    const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
+    void *k = pick_kernel(width, c.lds_stack, 2, any);
+    uint32_t grid = persistent_grid(ctx, k, c.block, lds);
+    void *args[] = {&a};
+    RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
+    RT_HIP(hipGetLastError());
+    RT_HIP(hipStreamSynchronize(stream));  // the item table is freed on return
+}
+
+}  // namespace rtgpu

@@ -1,0 +1,327 @@
+"""ctypes binding of librtgpu.so (include/rtgpu.h) and small device-memory helpers.
+
+The HIP library is the only compute path: if it is missing this module raises
+on import, there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import layouts as L
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("RTGPU_LIB", _PKG / "librtgpu.so"))
+
+if not LIB_PATH.exists():
+    raise ImportError(f"{LIB_PATH} not built: run `python -m raytracergpu_mastersproject_amd.build`")
+
+lib = C.CDLL(str(LIB_PATH))
+
+RT_OK = 0
+STATUS = {0: "RT_OK", 1: "RT_ERR_INVALID", 2: "RT_ERR_NO_MEMORY", 3: "RT_ERR_HIP", 4: "RT_ERR_NOT_FOUND",
+          5: "RT_ERR_BUDGET", 6: "RT_ERR_OVERFLOW", 7: "RT_ERR_BUSY"}
+
+BUILD_REFERENCE_LEAVES = 1
+BUILD_REFERENCE_MORTON = 2
+BUILD_KEEP_BINARY = 4
+TRACE_CLOSEST = 0
+TRACE_ANY = 1
+RENDER_REFERENCE_MATERIALS = 1
+RENDER_NO_PRIMARY_CACHE = 2
+
+
+class BuildOptions(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("width", C.c_uint32), ("leaf_size", C.c_uint32), ("_reserved", C.c_uint32)]
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("block", C.c_uint32), ("lds_stack", C.c_uint32), ("persistent", C.c_uint32), ("refill", C.c_uint32)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("ntri", C.c_uint32), ("nsph", C.c_uint32), ("nmat", C.c_uint32), ("nprim", C.c_uint32),
+                ("width", C.c_uint32), ("leaf_size", C.c_uint32), ("nwide", C.c_uint32), ("wide_depth", C.c_uint32),
+                ("stack_bound", C.c_uint32), ("flags", C.c_uint32), ("device_bytes", C.c_uint64),
+                ("build_ms", C.c_float), ("_pad", C.c_float)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("scene_id", C.c_uint32), ("nrays", C.c_uint32), ("rays", C.c_void_p), ("hits", C.c_void_p)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("cfg", LaunchConfig), ("ms", C.c_float), ("rays_per_s", C.c_float), ("tried", C.c_uint32),
+                ("_pad", C.c_uint32)]
+
+
+_vp, _u32, _i32, _sz = C.c_void_p, C.c_uint32, C.c_int, C.c_size_t
+_SIGS = {
+    "rt_create": (_i32, [C.POINTER(_vp), _i32]),
+    "rt_destroy": (None, [_vp]),
+    "rt_last_error": (C.c_char_p, [_vp]),
+    "rt_scene_upload": (_i32, [_vp, C.POINTER(_u32), _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
+                               C.POINTER(BuildOptions)]),
+    "rt_scene_release": (_i32, [_vp, _u32]),
+    "rt_scene_info_get": (_i32, [_vp, _u32, C.POINTER(SceneInfo)]),
+    "rt_scene_get_nodes": (_i32, [_vp, _u32, _vp, _u32]),
+    "rt_scene_get_slots": (_i32, [_vp, _u32, _vp, _u32]),
+    "rt_scene_get_wide": (_i32, [_vp, _u32, _vp, _sz]),
+    "rt_trace": (_i32, [_vp, _u32, _vp, _u32, _vp, _vp, _u32, C.POINTER(LaunchConfig), _vp]),
+    "rt_accum_clear": (_i32, [_vp, _vp, _u32, _vp]),
+    "rt_render": (_i32, [_vp, _u32, _vp, _u32, _u32, _u32, _u32, _vp, _vp, C.POINTER(LaunchConfig), _vp]),
+    "rt_resolve": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp]),
+    "rt_trace_batch": (_i32, [_vp, C.POINTER(BatchItem), _u32, _u32, C.POINTER(LaunchConfig), _vp]),
+    "rt_autotune": (_i32, [_vp, _u32, _vp, _u32, _vp, _u32, C.POINTER(TuneResult)]),
+    "rt_util_malloc": (_i32, [C.POINTER(_vp), _sz]),
+    "rt_util_free": (_i32, [_vp]),
+    "rt_util_memcpy_htod": (_i32, [_vp, _vp, _sz]),
+    "rt_util_memcpy_dtoh": (_i32, [_vp, _vp, _sz]),
+    "rt_util_memset": (_i32, [_vp, _i32, _sz]),
+    "rt_util_synchronize": (_i32, []),
+    "rt_util_stream_create": (_i32, [C.POINTER(_vp)]),
+    "rt_util_stream_destroy": (_i32, [_vp]),
+    "rt_util_stream_synchronize": (_i32, [_vp]),
+    "rt_util_event_create": (_i32, [C.POINTER(_vp)]),
+    "rt_util_event_destroy": (_i32, [_vp]),
+    "rt_util_event_record": (_i32, [_vp, _vp]),
+    "rt_util_event_elapsed_ms": (_i32, [_vp, _vp, C.POINTER(C.c_float)]),
+    "rt_util_device_count": (_i32, [C.POINTER(_i32)]),
+    "rt_util_device_name": (_i32, [_i32, C.c_char_p, _sz]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+EXPORTED = tuple(_SIGS)
+
+
+class RTError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _check(status, ctx=None, what=""):
+    if status != RT_OK:
+        msg = lib.rt_last_error(ctx).decode() if ctx else what
+        raise RTError(status, msg or what)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    if isinstance(a, np.ndarray):
+        assert a.flags.c_contiguous
+        return a.ctypes.data
+    return a
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    if lib.rt_util_device_count(C.byref(n)) != RT_OK:
+        return 0
+    return n.value
+
+
+def device_name(dev: int = 0) -> str:
+    buf = C.create_string_buffer(256)
+    _check(lib.rt_util_device_name(dev, buf, 256), what="device name")
+    return buf.value.decode()
+
+
+def synchronize():
+    _check(lib.rt_util_synchronize(), what="hipDeviceSynchronize")
+
+
+class DeviceArray:
+    """A device allocation made through the library (hipMalloc)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib.rt_util_malloc(C.byref(p), max(self.nbytes, 1)), what=f"malloc {nbytes}")
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, arr: np.ndarray) -> "DeviceArray":
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.nbytes)
+        if arr.nbytes:
+            _check(lib.rt_util_memcpy_htod(d.ptr, arr.ctypes.data, arr.nbytes), what="memcpy htod")
+        return d
+
+    def upload(self, arr: np.ndarray, offset: int = 0):
+        arr = np.ascontiguousarray(arr)
+        assert offset + arr.nbytes <= self.nbytes
+        if arr.nbytes:
+            _check(lib.rt_util_memcpy_htod(self.ptr + offset, arr.ctypes.data, arr.nbytes), what="memcpy htod")
+
+    def to_numpy(self, dtype, count: int | None = None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _check(lib.rt_util_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), what="memcpy dtoh")
+        return out
+
+    def zero(self):
+        _check(lib.rt_util_memset(self.ptr, 0, self.nbytes), what="memset")
+
+    def free(self):
+        if self.ptr:
+            lib.rt_util_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        _check(lib.rt_util_stream_create(C.byref(s)), what="stream create")
+        self.handle = s.value
+
+    def synchronize(self):
+        _check(lib.rt_util_stream_synchronize(self.handle), what="stream sync")
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.rt_util_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        _check(lib.rt_util_event_create(C.byref(e)), what="event create")
+        self.handle = e.value
+
+    def record(self, stream=None):
+        _check(lib.rt_util_event_record(self.handle, stream.handle if stream else None), what="event record")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        _check(lib.rt_util_event_elapsed_ms(self.handle, end.handle, C.byref(ms)), what="event elapsed")
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.rt_util_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def launch_config(block=0, lds_stack=0, persistent=1, refill=0):
+    return LaunchConfig(block, lds_stack, persistent, refill)
+
+
+class Context:
+    """One rt_context on one GPU (the replacement of the reference's Device + Raytracer state)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib.rt_create(C.byref(h), device), what=f"rt_create(device={device})")
+        self.h = h.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.rt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, status):
+        _check(status, self.h)
+
+    # ---- scenes ----
+    def upload(self, tris=None, spheres=None, materials=None, models=None, nodes=None, flags=0, width=4,
+               leaf_size=1) -> int:
+        tris = np.ascontiguousarray(tris if tris is not None else np.zeros(0, L.TRIANGLE), dtype=L.TRIANGLE)
+        spheres = np.ascontiguousarray(spheres if spheres is not None else np.zeros(0, L.SPHERE), dtype=L.SPHERE)
+        materials = np.ascontiguousarray(materials if materials is not None else np.zeros(0, L.MATERIAL),
+                                         dtype=L.MATERIAL)
+        if models is not None:
+            models = np.ascontiguousarray(models, dtype=np.float32).reshape(-1, 16)
+        if nodes is not None:
+            nodes = np.ascontiguousarray(nodes, dtype=L.BVH_NODE)
+        opts = BuildOptions(flags, width, leaf_size, 0)
+        sid = C.c_uint32()
+        self._ck(lib.rt_scene_upload(self.h, C.byref(sid), _ptr(tris), len(tris), _ptr(spheres), len(spheres),
+                                     _ptr(materials), len(materials), _ptr(models),
+                                     0 if models is None else len(models), _ptr(nodes),
+                                     0 if nodes is None else len(nodes), C.byref(opts)))
+        return sid.value
+
+    def release(self, sid: int):
+        self._ck(lib.rt_scene_release(self.h, sid))
+
+    def info(self, sid: int) -> SceneInfo:
+        inf = SceneInfo()
+        self._ck(lib.rt_scene_info_get(self.h, sid, C.byref(inf)))
+        return inf
+
+    def nodes(self, sid: int) -> np.ndarray:
+        n = self.info(sid).nprim
+        out = np.zeros(max(2 * n - 1, 0), dtype=L.BVH_NODE)
+        self._ck(lib.rt_scene_get_nodes(self.h, sid, out.ctypes.data, len(out)))
+        return out
+
+    def slots(self, sid: int) -> np.ndarray:
+        n = self.info(sid).nprim
+        out = np.zeros(n, dtype=np.uint32)
+        self._ck(lib.rt_scene_get_slots(self.h, sid, out.ctypes.data, n))
+        return out
+
+    def wide(self, sid: int) -> np.ndarray:
+        inf = self.info(sid)
+        out = np.zeros(inf.nwide * 8 * inf.width, dtype=np.uint32)
+        self._ck(lib.rt_scene_get_wide(self.h, sid, out.ctypes.data, out.nbytes))
+        return out
+
+    # ---- ray queries ----
+    def trace(self, sid, rays: DeviceArray, n: int, hits: DeviceArray, counts: DeviceArray | None = None,
+              any_hit=False, cfg: LaunchConfig | None = None, stream: Stream | None = None):
+        self._ck(lib.rt_trace(self.h, sid, _ptr(rays), n, _ptr(hits), _ptr(counts),
+                              TRACE_ANY if any_hit else TRACE_CLOSEST, C.byref(cfg) if cfg else None,
+                              stream.handle if stream else None))
+
+    def trace_batch(self, items, any_hit=False, cfg=None, stream=None):
+        arr = (BatchItem * len(items))(*[BatchItem(s, n, _ptr(r), _ptr(h)) for (s, n, r, h) in items])
+        self._ck(lib.rt_trace_batch(self.h, arr, len(items), TRACE_ANY if any_hit else TRACE_CLOSEST,
+                                    C.byref(cfg) if cfg else None, stream.handle if stream else None))
+
+    def autotune(self, sid, rays, n, hits, any_hit=False) -> TuneResult:
+        res = TuneResult()
+        self._ck(lib.rt_autotune(self.h, sid, _ptr(rays), n, _ptr(hits), TRACE_ANY if any_hit else 0, C.byref(res)))
+        return res
+
+    # ---- path tracing ----
+    def accum_clear(self, accum: DeviceArray, npix: int, stream=None):
+        self._ck(lib.rt_accum_clear(self.h, _ptr(accum), npix, stream.handle if stream else None))
+
+    def render(self, sid, params: np.ndarray, width, height, spp, accum: DeviceArray, flags=0,
+               segments: DeviceArray | None = None, cfg=None, stream=None):
+        params = np.ascontiguousarray(params, dtype=L.PARAMS)
+        self._ck(lib.rt_render(self.h, sid, params.ctypes.data, width, height, spp, flags, _ptr(accum),
+                               _ptr(segments), C.byref(cfg) if cfg else None, stream.handle if stream else None))
+
+    def resolve(self, accum: DeviceArray, npix: int, spp: int, out: DeviceArray, stream=None):
+        self._ck(lib.rt_resolve(self.h, _ptr(accum), npix, spp, _ptr(out), stream.handle if stream else None))

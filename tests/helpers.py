@@ -1,0 +1,59 @@
+"""Shared helpers for the parity tests: upload a scene to the HIP library and mirror it in the oracle."""
+import numpy as np
+
+from raytracergpu_mastersproject_amd import layouts as L
+from raytracergpu_mastersproject_amd import workloads as W
+
+
+def small_cluster_scene(nclusters=16, per_cluster=2048, seed=11):
+    tris, mats = W.cluster_scene(nclusters, per_cluster, seed=seed)
+    return tris, mats
+
+
+def with_spheres(tris, mats, nsph=5, seed=5):
+    rng = W.SplitMix64(seed)
+    sph = np.zeros(nsph, dtype=L.SPHERE)
+    sph["center"][:, :3] = rng.uniform(3 * nsph, -0.8, 0.8).reshape(nsph, 3)
+    sph["center"][:, 3] = 1.0
+    sph["radius"] = rng.uniform(nsph, 0.05, 0.2)
+    sph["materialIndex"] = 0
+    return sph
+
+
+def upload_both(ctx, O, tris, spheres=None, mats=None, flags=0, width=4, leaf_size=1):
+    """Build on the GPU (keeping the binary builder output) and return (sid, oracle scene over the
+    GPU's own wide BVH and slot table)."""
+    from raytracergpu_mastersproject_amd import native as N
+    if mats is None:
+        mats = W.make_materials([0.5, 0.5, 0.5], L.DIFFUSE)
+    sid = ctx.upload(tris, spheres, mats, flags=flags | N.BUILD_KEEP_BINARY, width=width, leaf_size=leaf_size)
+    nodes = ctx.nodes(sid)
+    slots = ctx.slots(sid)
+    wide = ctx.wide(sid)
+    osc = O.OracleScene(tris, spheres, mats, nodes.view(O.NODE), slots, wide, width)
+    return sid, osc
+
+
+def gpu_trace(ctx, sid, rays, any_hit=False, counts=False, cfg=None):
+    from raytracergpu_mastersproject_amd import native as N
+    n = len(rays)
+    dr = N.DeviceArray.from_numpy(rays)
+    dh = N.DeviceArray(16 * max(n, 1))
+    dc = N.DeviceArray(8 * max(n, 1)) if counts else None
+    ctx.trace(sid, dr, n, dh, dc, any_hit=any_hit, cfg=cfg)
+    hits = dh.to_numpy(L.HIT, n)
+    if counts:
+        c = dc.to_numpy(np.uint32, 2 * n).reshape(n, 2)
+        return hits, c
+    return hits
+
+
+def compare_hits(g, o, check_uv=True):
+    """Bit-exact comparison of closest-hit records; returns number of mismatching rays."""
+    bad = g["prim"] != o["prim"]
+    hit = o["prim"] != 0xFFFFFFFF
+    bad |= hit & (g["t"].view(np.uint32) != o["t"].view(np.uint32))
+    if check_uv:
+        bad |= hit & (g["u"].view(np.uint32) != o["u"].view(np.uint32))
+        bad |= hit & (g["v"].view(np.uint32) != o["v"].view(np.uint32))
+    return int(bad.sum())

@@ -1,0 +1,118 @@
+"""Path tracing on the GPU vs the oracle's restatement of raytraceBVH.comp (bit-exact images)."""
+import numpy as np
+import pytest
+
+from raytracergpu_mastersproject_amd import layouts as L
+from raytracergpu_mastersproject_amd import workloads as W
+
+from helpers import upload_both, with_spheres
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_render(native, ctx, sid, params, w, h, spp, flags=0, accum=None, cfg=None):
+    npix = w * h
+    acc = native.DeviceArray(16 * npix)
+    if accum is None:
+        ctx.accum_clear(acc, npix)
+    else:
+        acc.upload(accum)
+    segs = native.DeviceArray(8)
+    segs.zero()
+    ctx.render(sid, params, w, h, spp, acc, flags=flags, segments=segs, cfg=cfg)
+    out = acc.to_numpy(np.float32, 4 * npix).reshape(npix, 4)
+    return out, int(segs.to_numpy(np.uint64, 1)[0])
+
+
+def test_soup_image_bit_exact(ctx, native, O):
+    """The soup scene (1,024 triangles + ground + light), 256x256, 1 spp, depth 8: every pixel's
+    accumulated RGBA (including the RNG carry in alpha) equals the oracle's bit for bit."""
+    s = W.soup_scene()
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    g, gsegs = gpu_render(native, ctx, sid, s["params"], 256, 256, 1)
+    o, osegs = osc.render(s["params"], 256, 256, 1)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    assert gsegs == osegs  # spp=1: no cached primaries
+    rmse = float(np.sqrt(np.mean((g[:, :3] - o[:, :3]) ** 2)))
+    assert rmse < 1e-4
+    assert np.mean(g[:, 0] > 0) > 0.01
+    ctx.release(sid)
+
+
+@pytest.mark.parametrize("cache", [True, False])
+def test_spp_chain_and_primary_cache(ctx, native, O, cache):
+    s = W.soup_scene(width=64, height=48)
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    flags = 0 if cache else native.RENDER_NO_PRIMARY_CACHE
+    g4, segs4 = gpu_render(native, ctx, sid, s["params"], 64, 48, 4, flags)
+    o4, osegs = osc.render(s["params"], 64, 48, 4)
+    assert np.array_equal(g4.view(np.uint32), o4.view(np.uint32))
+    assert segs4 == (osegs - 64 * 48 * 3 if cache else osegs)
+    # two successive calls of 2 spp continue the per-pixel chain exactly like one call of 4
+    g2, _ = gpu_render(native, ctx, sid, s["params"], 64, 48, 2, flags)
+    g22, _ = gpu_render(native, ctx, sid, s["params"], 64, 48, 2, flags, accum=g2)
+    assert np.array_equal(g22.view(np.uint32), g4.view(np.uint32))
+    ctx.release(sid)
+
+
+def _material_scene():
+    s = W.icosphere_scene(freq=12, width=96, height=64, spp=2, depth=6)
+    return s
+
+
+@pytest.mark.parametrize("ref_mats", [False, True])
+def test_metal_dielectric_scene(ctx, native, O, ref_mats):
+    s = _material_scene()
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    flags = native.RENDER_REFERENCE_MATERIALS if ref_mats else 0
+    g, _ = gpu_render(native, ctx, sid, s["params"], 96, 64, 2, flags)
+    o, _ = osc.render(s["params"], 96, 64, 2, flags=O.RENDER_REFERENCE_MATERIALS if ref_mats else 0)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    ctx.release(sid)
+
+
+def test_spheres_scene(ctx, native, O):
+    s = W.soup_scene(ntri=300, width=80, height=60)
+    sph = with_spheres(s["tris"], s["materials"], 6)
+    sid, osc = upload_both(ctx, O, s["tris"], sph, s["materials"])
+    g, _ = gpu_render(native, ctx, sid, s["params"], 80, 60, 2)
+    o, _ = osc.render(s["params"], 80, 60, 2)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    ctx.release(sid)
+
+
+def test_rows_beyond_600_and_odd_sizes(ctx, native, O):
+    """Seed aliasing at rows >= 600 (random.glsl:10 uses a literal 600) and non-multiple-of-8 sizes."""
+    s = W.soup_scene(ntri=128)
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    w, h = 13, 611
+    g, _ = gpu_render(native, ctx, sid, s["params"], w, h, 1)
+    o, _ = osc.render(s["params"], w, h, 1)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    ctx.release(sid)
+
+
+def test_resolve(ctx, native, O):
+    rng = np.random.default_rng(0)
+    acc = rng.uniform(-0.5, 40.0, (1000, 4)).astype(np.float32)
+    da = native.DeviceArray.from_numpy(acc)
+    do = native.DeviceArray(acc.nbytes)
+    ctx.resolve(da, 1000, 7, do)
+    g = do.to_numpy(np.float32, 4000).reshape(1000, 4)
+    o = O.resolve(acc, 7)
+    assert np.array_equal(np.nan_to_num(g, nan=-1.0).view(np.uint32), np.nan_to_num(o, nan=-1.0).view(np.uint32))
+
+
+def test_depth_zero_and_empty_scene(ctx, native, O):
+    s = W.soup_scene(ntri=64, width=16, height=16)
+    p = s["params"].copy()
+    p["maxRayTraceDepth"] = 0
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    g, segs = gpu_render(native, ctx, sid, p, 16, 16, 3)
+    o, _ = osc.render(p, 16, 16, 3)
+    assert segs == 0 and np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    ctx.release(sid)
+    sid = ctx.upload(None, None, s["materials"])
+    g, _ = gpu_render(native, ctx, sid, s["params"], 16, 16, 2)
+    assert np.all(g[:, :3] == 0.0)
+    ctx.release(sid)

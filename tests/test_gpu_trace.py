@@ -1,0 +1,166 @@
+"""Ray queries on the GPU vs the oracle.
+
+Bar: closest-hit primitive ids, t, u, v bit-exact against (a) the oracle's brute force in the
+reference's visit order (right-first: among equal t the smallest leaf slot wins) and (b) the
+oracle's replay of the same wide traversal, which must also reproduce the kernel's per-ray
+AABB-test and primitive-test counts exactly.  Any-hit: existence equals brute force.
+"""
+import numpy as np
+import pytest
+
+from raytracergpu_mastersproject_amd import layouts as L
+from raytracergpu_mastersproject_amd import workloads as W
+
+from helpers import compare_hits, gpu_trace, small_cluster_scene, upload_both, with_spheres
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = [
+    dict(block=128, lds_stack=8, persistent=1, refill=16),
+    dict(block=64, lds_stack=4, persistent=0, refill=0),
+    dict(block=256, lds_stack=16, persistent=1, refill=64),
+]
+
+
+@pytest.mark.parametrize("width,leaf", [(4, 1), (2, 1), (8, 1), (4, 4), (8, 2)])
+def test_closest_hit_clusters(ctx, native, O, width, leaf):
+    tris, mats = small_cluster_scene(16, 4096)
+    sid, osc = upload_both(ctx, O, tris, None, mats, width=width, leaf_size=leaf)
+    lo, hi = W.scene_bounds(tris)
+    rays = W.incoherent_rays(40000, lo, hi, seed=21)
+    g, gc = gpu_trace(ctx, sid, rays, counts=True)
+    ow, oc = osc.trace(rays, O.WIDE, counts=True)
+    assert compare_hits(g, ow) == 0
+    assert np.array_equal(gc[:, 0], oc["n_box"]) and np.array_equal(gc[:, 1], oc["n_tri"])
+    ob = osc.trace(rays[:4000], O.BRUTE)
+    assert compare_hits(g[:4000], ob) == 0
+    assert np.mean(g["prim"] != L.NO_HIT) > 0.05
+    ctx.release(sid)
+
+
+@pytest.mark.parametrize("cfg", CONFIGS)
+def test_launch_configs_agree(ctx, native, O, cfg):
+    tris, mats = small_cluster_scene(8, 4096, seed=3)
+    sid, osc = upload_both(ctx, O, tris, None, mats)
+    lo, hi = W.scene_bounds(tris)
+    rays = W.incoherent_rays(30011, lo, hi, seed=22)
+    g = gpu_trace(ctx, sid, rays, cfg=native.launch_config(**cfg))
+    assert compare_hits(g, osc.trace(rays, O.WIDE)) == 0
+    ctx.release(sid)
+
+
+def test_soup_scene_vs_reference_traversal(ctx, native, O):
+    """C1 geometry: GPU == brute force == the reference's own binary LBVH traversal."""
+    s = W.soup_scene()
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    lo, hi = W.scene_bounds(s["tris"])
+    rays = W.incoherent_rays(50000, lo, hi, seed=23)
+    g = gpu_trace(ctx, sid, rays)
+    assert compare_hits(g, osc.trace(rays, O.BRUTE)) == 0
+    assert compare_hits(g, osc.trace(rays, O.REF_BVH)) == 0
+    ctx.release(sid)
+
+
+def test_any_hit(ctx, native, O):
+    tris, mats = small_cluster_scene(16, 2048, seed=4)
+    sid, osc = upload_both(ctx, O, tris, None, mats)
+    lo, hi = W.scene_bounds(tris)
+    rays = W.incoherent_rays(30000, lo, hi, seed=24)
+    rays["tmax"] = np.where(np.arange(len(rays)) % 3 == 0, 0.3, 1e7).astype(np.float32)  # short shadow rays
+    g, gc = gpu_trace(ctx, sid, rays, any_hit=True, counts=True)
+    ob = osc.trace(rays, O.BRUTE, any_hit=True)
+    assert np.array_equal(g["prim"] != L.NO_HIT, ob["prim"] != L.NO_HIT)
+    ow, oc = osc.trace(rays, O.WIDE, any_hit=True, counts=True)
+    assert np.array_equal(g["prim"], ow["prim"]) and np.array_equal(gc[:, 0], oc["n_box"])
+    hit = g["prim"] != L.NO_HIT
+    assert np.all((g["t"][hit] >= rays["tmin"][hit]) & (g["t"][hit] <= rays["tmax"][hit]))
+    ctx.release(sid)
+
+
+def test_spheres_and_triangles(ctx, native, O):
+    tris, mats = small_cluster_scene(4, 1000, seed=6)
+    sph = with_spheres(tris, mats, 12)
+    sid, osc = upload_both(ctx, O, tris, sph, mats)
+    lo, hi = W.scene_bounds(tris)
+    rays = W.incoherent_rays(20000, lo, hi, seed=25)
+    g = gpu_trace(ctx, sid, rays)
+    ob = osc.trace(rays, O.BRUTE)
+    assert compare_hits(g, ob, check_uv=False) == 0
+    assert np.any((g["prim"] & L.SPHERE_BIT) != 0) and np.any(g["prim"] == ob["prim"])
+    ctx.release(sid)
+
+
+def test_edge_rays(ctx, native, O):
+    tris, mats = small_cluster_scene(2, 100, seed=8)
+    sid, osc = upload_both(ctx, O, tris, None, mats)
+    r = np.zeros(6, L.RAY)
+    r["o"] = [0, 0, -5]
+    r["d"] = [0, 0, 1]
+    r["tmin"], r["tmax"] = 0.001, 1e7
+    r[1]["d"] = [np.nan, 0, 1]            # NaN direction: miss
+    r[2]["tmin"], r[2]["tmax"] = 5.0, 1.0  # empty interval: miss
+    r[3]["d"] = [0, 0, 0]                 # zero direction
+    r[4]["d"] = [1e-30, 0, 1]             # near-axis direction
+    r[5]["o"] = [0, 0, 0]                 # origin inside the geometry
+    g = gpu_trace(ctx, sid, r)
+    assert g[1]["prim"] == L.NO_HIT and g[2]["prim"] == L.NO_HIT
+    assert compare_hits(g, osc.trace(r, O.WIDE)) == 0
+    assert compare_hits(g, osc.trace(r, O.BRUTE)) == 0
+    # zero rays is a no-op
+    dh = native.DeviceArray(16)
+    ctx.trace(sid, dh, 0, dh)
+    ctx.release(sid)
+
+
+def test_empty_scene_misses(ctx, native):
+    sid = ctx.upload(None, None, W.make_materials([1, 1, 1], L.DIFFUSE))
+    rays = W.incoherent_rays(1000, [-1, -1, -1], [1, 1, 1], seed=1)
+    g = gpu_trace(ctx, sid, rays)
+    assert np.all(g["prim"] == L.NO_HIT) and np.array_equal(g["t"], rays["tmax"])
+    ctx.release(sid)
+
+
+def test_ties_resolve_to_smallest_slot(ctx, native, O):
+    """Duplicate triangles give exactly equal t; the reference keeps the one it tests last."""
+    tris, mats = small_cluster_scene(2, 300, seed=9)
+    tris = np.concatenate([tris, tris])
+    sid, osc = upload_both(ctx, O, tris, None, mats)
+    lo, hi = W.scene_bounds(tris)
+    rays = W.incoherent_rays(20000, lo, hi, seed=26)
+    g = gpu_trace(ctx, sid, rays)
+    assert compare_hits(g, osc.trace(rays, O.BRUTE)) == 0
+    assert compare_hits(g, osc.trace(rays, O.REF_BVH)) == 0
+    ctx.release(sid)
+
+
+def test_batched_launch_equals_single(ctx, native, O):
+    items, ref = [], []
+    keep = []
+    for k in range(5):
+        tris, mats = small_cluster_scene(4, 500 * (k + 1), seed=30 + k)
+        sid = ctx.upload(tris, None, mats)
+        lo, hi = W.scene_bounds(tris)
+        rays = W.incoherent_rays(3000 + 1000 * k, lo, hi, seed=40 + k)
+        dr = native.DeviceArray.from_numpy(rays)
+        dh = native.DeviceArray(16 * len(rays))
+        items.append((sid, len(rays), dr, dh))
+        ref.append(gpu_trace(ctx, sid, rays))
+        keep.append((dr, dh))
+    ctx.trace_batch(items)
+    for (sid, n, dr, dh), r in zip(items, ref):
+        assert compare_hits(dh.to_numpy(L.HIT, n), r) == 0
+        ctx.release(sid)
+
+
+def test_autotune_sets_a_working_default(ctx, native, O):
+    tris, mats = small_cluster_scene(8, 2048, seed=12)
+    sid, osc = upload_both(ctx, O, tris, None, mats)
+    lo, hi = W.scene_bounds(tris)
+    rays = W.incoherent_rays(20000, lo, hi, seed=27)
+    dr = native.DeviceArray.from_numpy(rays)
+    dh = native.DeviceArray(16 * len(rays))
+    res = ctx.autotune(sid, dr, len(rays), dh)
+    assert res.tried > 0 and res.ms > 0
+    g = gpu_trace(ctx, sid, rays)
+    assert compare_hits(g, osc.trace(rays, O.WIDE)) == 0
+    ctx.release(sid)
