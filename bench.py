@@ -69,6 +69,19 @@ def cpu_baseline(tris, mats, rays, seconds: float, threads: int):
                       f"(RaytracerBVH reference algorithm), {threads} threads; CPU LBVH build {build_s:.1f} s"}
 
 
+def copy_calibration(mib: int = 1024, reps: int = 5):
+    """Device-to-device copies of a known size (hipMemcpy's copy kernel): the reference for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes."""
+    from raytracergpu_mastersproject_amd import native as N
+    n = mib << 20
+    a, b = N.DeviceArray(n), N.DeviceArray(n)
+    a.zero()
+    for _ in range(reps):
+        N.copy_dtod(b, a, n)
+    N.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_dispatch": n, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,9 +93,15 @@ def main():
     ap.add_argument("--width", type=int, default=int(os.environ.get("RTGPU_WIDTH", 4)))
     ap.add_argument("--leaf", type=int, default=int(os.environ.get("RTGPU_LEAF", 1)))
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cfg", type=str, default="",
+                    help="fixed launch config block,lds_stack,persistent,refill (skips autotuning)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true", help="only run the PMC calibration copies")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is designed but not "
@@ -114,14 +133,20 @@ def main():
     d_counts = N.DeviceArray(8 * n)
     stream = N.Stream()
 
-    tune = None
-    if not args.no_autotune:
+    tune = tune_any = None
+    fixed = None
+    if args.cfg:
+        b, s_, p_, r_ = (int(x) for x in args.cfg.split(","))
+        fixed = N.launch_config(b, s_, p_, r_)
+    elif not args.no_autotune:
         tune = ctx.autotune(sid, d_rays, n, d_hits)
-        log(f"autotune: block {tune.cfg.block} lds_stack {tune.cfg.lds_stack} persistent {tune.cfg.persistent} "
-            f"refill {tune.cfg.refill} -> {tune.ms:.2f} ms ({tune.tried} configs)")
+        tune_any = ctx.autotune(sid, d_rays, n, d_hits, any_hit=True)
+        for nm, t in (("closest", tune), ("any", tune_any)):
+            log(f"autotune {nm}: block {t.cfg.block} lds_stack {t.cfg.lds_stack} persistent {t.cfg.persistent} "
+                f"refill {t.cfg.refill} -> {t.ms:.2f} ms ({t.tried} configs)")
 
     # per-ray AABB / triangle test counts (untimed stats launch of the same traversal)
-    ctx.trace(sid, d_rays, n, d_hits, d_counts, stream=stream)
+    ctx.trace(sid, d_rays, n, d_hits, d_counts, cfg=fixed, stream=stream)
     stream.synchronize()
     counts = d_counts.to_numpy(np.uint32, 2 * n).reshape(n, 2)
     hits = d_hits.to_numpy(L.HIT, n)
@@ -131,7 +156,7 @@ def main():
         f"hit fraction {hit_frac:.3f}")
 
     for _ in range(args.warmup):
-        ctx.trace(sid, d_rays, n, d_hits, stream=stream)
+        ctx.trace(sid, d_rays, n, d_hits, cfg=fixed, stream=stream)
     stream.synchronize()
 
     ev = [N.Event() for _ in range(args.steps + 1)]
@@ -139,7 +164,7 @@ def main():
     t_wall = time.perf_counter()
     ev[0].record(stream)
     for k in range(args.steps):
-        ctx.trace(sid, d_rays, n, d_hits, stream=stream)
+        ctx.trace(sid, d_rays, n, d_hits, cfg=fixed, stream=stream)
         ev[k + 1].record(stream)
     stream.synchronize()
     N.synchronize()
@@ -152,10 +177,10 @@ def main():
 
     # any-hit on the same rays
     ea, eb = N.Event(), N.Event()
-    ctx.trace(sid, d_rays, n, d_hits, any_hit=True, stream=stream)
+    ctx.trace(sid, d_rays, n, d_hits, any_hit=True, cfg=fixed, stream=stream)
     ea.record(stream)
     for _ in range(3):
-        ctx.trace(sid, d_rays, n, d_hits, any_hit=True, stream=stream)
+        ctx.trace(sid, d_rays, n, d_hits, any_hit=True, cfg=fixed, stream=stream)
     eb.record(stream)
     any_ms = ea.elapsed_ms(eb) / 3
 

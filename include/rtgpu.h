@@ -214,6 +214,7 @@ int rt_util_malloc(void **ptr, size_t bytes);
 int rt_util_free(void *ptr);
 int rt_util_memcpy_htod(void *dst, const void *src, size_t bytes);
 int rt_util_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int rt_util_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int rt_util_memset(void *ptr, int value, size_t bytes);
 int rt_util_synchronize(void);
 int rt_util_stream_create(void **stream);

@@ -600,6 +600,13 @@ static const float kRobust = 0x1.000006p+0f;
 
 typedef struct { uint32_t e; float tn; } wentry;
 
+/* order-preserving float -> uint32 map (total order, -0 < +0) */
+static inline uint32_t ord_key(float f) {
+    uint32_t b;
+    memcpy(&b, &f, sizeof b);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
 static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float tmax, hitrec *best,
                              orc_counts *cnt, int any) {
     const uint32_t W = sc->width;
@@ -652,13 +659,19 @@ static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float 
                 nh++;
             }
         }
-        if (!any) { /* stable insertion sort by entry distance */
+        if (!any) { /* ascending (order-preserving key of the entry distance, entry) */
             for (int i = 1; i < nh; ++i) {
                 wentry x = cand[i];
+                uint64_t kx = ((uint64_t)ord_key(x.tn) << 32) | x.e;
                 int j = i - 1;
-                while (j >= 0 && cand[j].tn > x.tn) { cand[j + 1] = cand[j]; j--; }
+                while (j >= 0 && (((uint64_t)ord_key(cand[j].tn) << 32) | cand[j].e) > kx) {
+                    cand[j + 1] = cand[j];
+                    j--;
+                }
                 cand[j + 1] = x;
             }
+        } else {
+            for (int i = 0; i < nh; ++i) cand[i].tn = -INFINITY; /* any-hit entries are never culled */
         }
         for (int i = nh - 1; i >= 0; --i) stack[sp++] = cand[i];
         if (sp > 1000) abort();

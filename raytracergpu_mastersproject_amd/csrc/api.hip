@@ -271,6 +271,9 @@ int rt_util_memcpy_htod(void *dst, const void *src, size_t bytes) {
 int rt_util_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
     return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
 }
+int rt_util_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+}
 int rt_util_memset(void *ptr, int value, size_t bytes) { return hip_status(hipMemset(ptr, value, bytes)); }
 int rt_util_synchronize(void) { return hip_status(hipDeviceSynchronize()); }
 int rt_util_stream_create(void **stream) {
@@ -299,7 +302,9 @@ int rt_util_device_name(int device, char *buf, size_t len) {
     hipDeviceProp_t prop;
     hipError_t e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) return hip_status(e);
-    std::snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+    char name[256] = {0};
+    if (hipDeviceGetName(name, sizeof name, device) != hipSuccess || !name[0]) std::snprintf(name, sizeof name, "%s", prop.name);
+    std::snprintf(buf, len, "%s (%s, %d CUs)", name[0] ? name : "AMD GPU", prop.gcnArchName, prop.multiProcessorCount);
     return RT_OK;
 }
 

@@ -57,7 +57,7 @@ struct RenderArgs {
     const uint32_t *slot_mat;
     const rt_material *mats;
     uint32_t nwide;
-    uint32_t width, height, spp, depth, random_state, flags, refill;
+    uint32_t width, height, spp, depth, random_state, flags, refill, leaf_weight;
     uint32_t tiles_x, nitems;
     float cam[12];
     float *accum;
@@ -66,6 +66,7 @@ struct RenderArgs {
 };
 
 enum : uint32_t { kIdle = 0, kTracing = 1, kResolved = 2 };
+constexpr uint32_t kRenderChunk = 128;  // pixels (two 8x8 tiles) per atomic on the work counter
 
 struct PixelState {
     uint32_t phase;
@@ -220,19 +221,26 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
     t.top = 0;
     uint32_t segs = 0;
     bool exhausted = false;
+    uint32_t q_next = 0, q_end = 0;  // the wave's current chunk of work items (wave-uniform)
     for (;;) {
         const unsigned long long idle = __ballot(ps.phase == kIdle);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (!exhausted && nidle >= a.refill) {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(a.work, nidle);
-            base = (uint32_t)__shfl((int)base, leader);
-            if ((uint64_t)base + nidle >= a.nitems) exhausted = true;
-            if (ps.phase == kIdle) {
-                const uint64_t item = (uint64_t)base + (uint64_t)__popcll(idle & lt);
+            if (q_next == q_end) {
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(a.work, kRenderChunk);
+                base = (uint32_t)__shfl((int)base, leader);
+                if (base >= a.nitems) exhausted = true;
+                else { q_next = base; q_end = (uint32_t)min((uint64_t)base + kRenderChunk, (uint64_t)a.nitems); }
+            }
+            const uint32_t take = exhausted ? 0u : min(nidle, q_end - q_next);
+            const uint32_t r = (uint32_t)__popcll(idle & lt);
+            const uint32_t item = q_next + r;
+            q_next += take;
+            if (ps.phase == kIdle && r < take) {
                 uint32_t x, y;
-                if (item < a.nitems && item_to_pixel(a, (uint32_t)item, x, y)) {
+                if (item_to_pixel(a, item, x, y)) {
                     ps.x = x;
                     ps.y = y;
                     ps.pix = y * a.width + x;
@@ -247,13 +255,17 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
             }
         }
         if (__ballot(ps.phase != kIdle) == 0ull) break;
-        if (ps.phase == kTracing) {
-            trav_step<W, S, false>(t, m, a.wide, a.rec);
-            if (trav_done(t)) {
-                ps.hslot = t.bslot;
-                ps.ht = t.bt;
-                ps.phase = kResolved;
-            }
+        const bool tracing = ps.phase == kTracing;
+        const bool on_node = tracing && at_node(t), on_leaf = tracing && at_leaf(t);
+        if (choose_leaf_phase(on_node, on_leaf, a.leaf_weight)) {
+            if (on_leaf) leaf_step<S, false>(t, m, a.rec);
+        } else {
+            if (on_node) node_step<W, S, false>(t, m, a.wide);
+        }
+        if (tracing && trav_done(t)) {
+            ps.hslot = t.bslot;
+            ps.ht = t.bt;
+            ps.phase = kResolved;
         }
         // resolved segments may chain (cached primary hit, depth reached): bounded by spp*depth
         while (ps.phase == kResolved) shade(a, ps, t, segs);
@@ -335,6 +347,7 @@ void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_
     a.random_state = p.randomState;
     a.flags = flags;
     a.refill = c.refill;
+    a.leaf_weight = c.leaf_weight;
     a.tiles_x = (width + 7) / 8;
     const uint64_t items = (uint64_t)a.tiles_x * ((height + 7) / 8) * 64u;
     a.nitems = (uint32_t)items;

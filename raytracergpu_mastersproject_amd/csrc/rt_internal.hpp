@@ -69,6 +69,7 @@ struct LaunchCfg {
     uint32_t lds_stack = 8;
     uint32_t persistent = 1;
     uint32_t refill = 16;
+    uint32_t leaf_weight = 2;  // leaf phase when leaf lanes * weight >= node lanes
 };
 
 // A scene resident in HBM.

@@ -8,6 +8,7 @@
 //   direct: one lane per ray (the plain baseline).
 // The batched variant runs many scenes' ray batches in one persistent launch.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "rt_internal.hpp"
@@ -32,6 +33,7 @@ struct TraceArgs {
     uint32_t n;
     uint32_t refill;
     unsigned int *work;
+    uint32_t leaf_weight;
 };
 
 __device__ __forceinline__ void load_ray(const rt_ray *rays, uint32_t i, v3 &o, float &tmin, v3 &d, float &tmax) {
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(256) void k_trace_direct(TraceArgs a) {
     load_ray(a.rays, i, o, tmin, d, tmax);
     Trav t;
     trav_init(t, o, d, tmin, tmax, a.sc.nwide);
-    while (!trav_done(t)) trav_step<W, S, ANY>(t, m, a.sc.wide, a.sc.rec);
+    trav_run<W, S, ANY>(t, m, a.sc.wide, a.sc.rec, a.leaf_weight);
     store_hit(a.sc, t, a.hits, a.counts, i);
 }
 
@@ -77,6 +79,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
     int64_t rid = -1;
     bool exhausted = false;
+    uint32_t q_next = 0, q_end = 0;  // the wave's current chunk of ray indices (wave-uniform)
     Trav t;
     t.have = false;
     t.top = 0;
@@ -84,29 +87,37 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
         const unsigned long long idle = __ballot(rid < 0);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (!exhausted && nidle >= a.refill) {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(a.work, nidle);
-            base = (uint32_t)__shfl((int)base, leader);
-            if ((uint64_t)base + nidle >= a.n) exhausted = true;
-            if (rid < 0) {
-                const uint64_t mine = (uint64_t)base + (uint64_t)__popcll(idle & lt);
-                if (mine < a.n) {
-                    rid = (int64_t)mine;
+            if (q_next == q_end) {  // one atomic per chunk keeps the shared counter off the critical path
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(a.work, kChunk);
+                base = (uint32_t)__shfl((int)base, leader);
+                if (base >= a.n) exhausted = true;
+                else { q_next = base; q_end = (uint32_t)min((uint64_t)base + kChunk, (uint64_t)a.n); }
+            }
+            if (!exhausted) {
+                const uint32_t take = min(nidle, q_end - q_next);
+                const uint32_t r = (uint32_t)__popcll(idle & lt);
+                if (rid < 0 && r < take) {
+                    rid = (int64_t)(q_next + r);
                     v3 o, d;
                     float tmin, tmax;
-                    load_ray(a.rays, (uint32_t)mine, o, tmin, d, tmax);
+                    load_ray(a.rays, (uint32_t)rid, o, tmin, d, tmax);
                     trav_init(t, o, d, tmin, tmax, a.sc.nwide);
                 }
+                q_next += take;
             }
         }
         if (__ballot(rid >= 0) == 0ull) break;  // every lane idle and the queue is drained
-        if (rid >= 0) {
-            trav_step<W, S, ANY>(t, m, a.sc.wide, a.sc.rec);
-            if (trav_done(t)) {
-                store_hit(a.sc, t, a.hits, a.counts, (uint32_t)rid);
-                rid = -1;
-            }
+        const bool on_node = rid >= 0 && at_node(t), on_leaf = rid >= 0 && at_leaf(t);
+        if (choose_leaf_phase(on_node, on_leaf, a.leaf_weight)) {
+            if (on_leaf) leaf_step<S, ANY>(t, m, a.sc.rec);
+        } else {
+            if (on_node) node_step<W, S, ANY>(t, m, a.sc.wide);
+        }
+        if (rid >= 0 && trav_done(t)) {
+            store_hit(a.sc, t, a.hits, a.counts, (uint32_t)rid);
+            rid = -1;
         }
     }
 }
@@ -129,6 +140,7 @@ struct BatchArgs {
     uint32_t refill;
     uint64_t total;
     unsigned long long *work;
+    uint32_t leaf_weight;
 };
 
 __device__ __forceinline__ uint32_t find_item(const BatchItemDev *items, uint32_t nitems, uint64_t g) {
@@ -150,6 +162,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
     bool active = false, exhausted = false;
     uint32_t item = 0, local = 0;
+    uint64_t q_next = 0, q_end = 0;  // the wave's current chunk of global ray indices
     Trav t;
     t.have = false;
     t.top = 0;
@@ -157,32 +170,38 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
         const unsigned long long idle = __ballot(!active);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (!exhausted && nidle >= a.refill) {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            unsigned long long base = 0;
-            if ((int)lane == leader) base = atomicAdd(a.work, (unsigned long long)nidle);
-            base = __shfl(base, leader);
-            if (base + nidle >= a.total) exhausted = true;
-            if (!active) {
-                const uint64_t g = base + (uint64_t)__popcll(idle & lt);
-                if (g < a.total) {
-                    item = find_item(a.items, a.nitems, g);
-                    local = (uint32_t)(g - a.items[item].first);
-                    v3 o, d;
-                    float tmin, tmax;
-                    load_ray(a.items[item].rays, local, o, tmin, d, tmax);
-                    trav_init(t, o, d, tmin, tmax, a.items[item].sc.nwide);
-                    active = true;
-                }
+            if (q_next == q_end) {
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                unsigned long long base = 0;
+                if ((int)lane == leader) base = atomicAdd(a.work, (unsigned long long)kChunk);
+                base = __shfl(base, leader);
+                if (base >= a.total) exhausted = true;
+                else { q_next = base; q_end = min((uint64_t)base + kChunk, a.total); }
+            }
+            const uint32_t take = exhausted ? 0u : (uint32_t)min((uint64_t)nidle, q_end - q_next);
+            const uint32_t r = (uint32_t)__popcll(idle & lt);
+            const uint64_t g = q_next + r;
+            q_next += take;
+            if (!active && r < take) {
+                item = find_item(a.items, a.nitems, g);
+                local = (uint32_t)(g - a.items[item].first);
+                v3 o, d;
+                float tmin, tmax;
+                load_ray(a.items[item].rays, local, o, tmin, d, tmax);
+                trav_init(t, o, d, tmin, tmax, a.items[item].sc.nwide);
+                active = true;
             }
         }
         if (__ballot(active) == 0ull) break;
-        if (active) {
-            const BatchItemDev &it = a.items[item];
-            trav_step<W, S, ANY>(t, m, it.sc.wide, it.sc.rec);
-            if (trav_done(t)) {
-                store_hit(it.sc, t, it.hits, nullptr, local);
-                active = false;
-            }
+        const bool on_node = active && at_node(t), on_leaf = active && at_leaf(t);
+        if (choose_leaf_phase(on_node, on_leaf, a.leaf_weight)) {
+            if (on_leaf) leaf_step<S, ANY>(t, m, a.items[item].sc.rec);
+        } else {
+            if (on_node) node_step<W, S, ANY>(t, m, a.items[item].sc.wide);
+        }
+        if (active && trav_done(t)) {
+            store_hit(a.items[item].sc, t, a.items[item].hits, nullptr, local);
+            active = false;
         }
     }
 }
@@ -249,6 +268,7 @@ LaunchCfg resolve_cfg(const rt_context *ctx, const Scene &sc, const rt_launch_co
     RT_REQUIRE(c.block == 64 || c.block == 128 || c.block == 256, "block must be 64, 128 or 256");
     RT_REQUIRE(c.lds_stack == 4 || c.lds_stack == 8 || c.lds_stack == 16, "lds_stack must be 4, 8 or 16");
     c.refill = std::min<uint32_t>(std::max<uint32_t>(c.refill, 1u), 64u);
+    if (const char *lw = std::getenv("RTGPU_LEAF_WEIGHT")) c.leaf_weight = (uint32_t)std::max(1, std::atoi(lw));
     return c;
 }
 
@@ -258,7 +278,7 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
     RT_REQUIRE(rays && hits, "rays/hits must be device pointers");
     check_stack(sc, c);
     const bool any = (flags & RT_TRACE_ANY) != 0;
-    TraceArgs a{view_of(sc), rays, hits, counts, n, c.refill, nullptr};
+    TraceArgs a{view_of(sc), rays, hits, counts, n, c.refill, nullptr, c.leaf_weight};
     const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
     if (!c.persistent) {
         void *k = pick_kernel(sc.width, c.lds_stack, 0, any);
@@ -297,7 +317,7 @@ void launch_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t ni
     RT_HIP(hipMemcpyAsync(ditems.p, hv.data(), sizeof(BatchItemDev) * nitems, hipMemcpyHostToDevice, stream));
     unsigned long long *work = reinterpret_cast<unsigned long long *>(ctx->counter_slot());
     k_zero_counter<<<1, 64, 0, stream>>>(reinterpret_cast<unsigned int *>(work));
-    BatchArgs a{ditems.as<BatchItemDev>(), nitems, c.refill, total, work};
+    BatchArgs a{ditems.as<BatchItemDev>(), nitems, c.refill, total, work, c.leaf_weight};
     const bool any = (flags & RT_TRACE_ANY) != 0;
     const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
     void *k = pick_kernel(width, c.lds_stack, 2, any);

@@ -1,18 +1,24 @@
-// traverse.hpp -- wide-BVH traversal step shared by the ray-query and path-tracing kernels.
+// traverse.hpp -- wide-BVH traversal shared by the ray-query and path-tracing kernels.
 //
-// One lane owns one ray.  Each call of step() processes one stack entry: a wide
-// node (W slab tests, hit children sorted near-to-far, nearest kept as the
-// current entry, the rest pushed) or a leaf (its primitives tested).  The
-// stack keeps its newest `S` entries in an LDS column private to the lane and
-// spills older ones to a private (scratch) array, so deep trees never fault.
+// One lane owns one ray.  A ray's current entry is either a wide node or a leaf:
+//   node_step(): W slab tests; hit children are ordered by (entry distance, entry)
+//                -- one 64-bit key each, sorted by a network -- the nearest becomes
+//                the current entry and the rest are pushed far-to-near;
+//   leaf_step(): the leaf's primitives are tested (triangleHit / sphereHit).
+// Each ray's sequence of visited entries depends only on the ray and the tree, so
+// its counts of AABB and primitive tests are reproducible.  The kernels choose, per
+// wave and per iteration, whether lanes sitting on nodes or lanes sitting on leaves
+// advance (a ballot); lanes of the other kind simply wait, which changes when a ray
+// advances but never the order of its own steps.  This keeps the ~80-instruction
+// triangle test from being executed by the whole wave on almost every iteration.
+//
+// The stack keeps its newest S entries in an LDS column private to the lane and
+// spills older ones to a private array, so deep trees never fault.
 //
 // Closest-hit semantics are those of hitBVH (raytraceBVH.comp:195-265) for any
-// traversal order: the smallest t wins and, among equal t, the smallest leaf
-// slot wins -- exactly the primitive the reference's right-first traversal
-// tests last, because it accepts equal t (`t > tMax` rejects, :130).
-//
-// The visit sequence of a ray depends only on the ray and the tree, never on
-// scheduling, so per-ray counts of AABB and primitive tests are reproducible.
+// traversal order: the smallest t wins and, among equal t, the smallest leaf slot
+// wins -- exactly the primitive the reference's right-first traversal tests last,
+// because it accepts equal t (`t > tMax` rejects, :130).
 #pragma once
 
 #include "rt_device.hpp"
@@ -20,6 +26,7 @@
 namespace rtgpu {
 
 constexpr uint32_t kSpillCap = 184;  // private spill entries per lane (beyond the LDS part)
+constexpr uint32_t kChunk = 512;     // work items a wave takes from the global counter per atomic
 
 template <int W> struct WideNode {
     float lo[3][W], hi[3][W];
@@ -62,6 +69,17 @@ template <> __device__ __forceinline__ void load_node<8>(const uint32_t *wide, u
     nd.child[4] = c1.x; nd.child[5] = c1.y; nd.child[6] = c1.z; nd.child[7] = c1.w;
 }
 
+// order-preserving map float -> uint32 (total order, -0 < +0) and back
+__device__ __forceinline__ uint32_t ord_key(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord_dec(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+constexpr uint32_t kOrdNegInf = 0x007FFFFFu;  // ord_key(-inf): never culled
+
 // per-lane traversal state
 struct Trav {
     v3 o, d, idir;
@@ -81,12 +99,13 @@ struct StackMem {
     uint2 *spill;     // private spill array (kSpillCap entries)
 };
 
-template <int S> __device__ __forceinline__ void push(Trav &t, const StackMem &m, uint32_t e, float tn) {
+// stack entries hold (entry, ord_key(entry distance))
+template <int S> __device__ __forceinline__ void push(Trav &t, const StackMem &m, uint32_t e, uint32_t tnkey) {
     if (t.top - t.nspill == (uint32_t)S) {  // LDS part full: move its oldest entry out
         if (t.nspill < kSpillCap) m.spill[t.nspill] = m.col[(t.nspill & (S - 1)) * m.stride];
         t.nspill++;
     }
-    m.col[(t.top & (S - 1)) * m.stride] = make_uint2(e, __float_as_uint(tn));
+    m.col[(t.top & (S - 1)) * m.stride] = make_uint2(e, tnkey);
     t.top++;
 }
 
@@ -101,7 +120,7 @@ template <int S> __device__ __forceinline__ void pop(Trav &t, const StackMem &m)
         v = (t.nspill < kSpillCap) ? m.spill[t.nspill] : make_uint2(kEmpty, 0u);
     }
     t.cur = v.x;
-    t.cur_tn = __uint_as_float(v.y);
+    t.cur_tn = ord_dec(v.y);
     t.have = v.x != kEmpty;
 }
 
@@ -124,99 +143,122 @@ __device__ __forceinline__ void trav_init(Trav &t, v3 o, v3 d, float tmin, float
 }
 
 __device__ __forceinline__ bool trav_done(const Trav &t) { return !t.have && t.top == 0; }
+__device__ __forceinline__ bool at_leaf(const Trav &t) { return t.have && (t.cur & kLeafBit); }
+__device__ __forceinline__ bool at_node(const Trav &t) { return t.have && !(t.cur & kLeafBit); }
 
-template <int W>
-__device__ __forceinline__ void cswap(float *k, uint32_t *ix, uint32_t *e, float *tn, int i, int j) {
-    bool sw = (k[i] > k[j]) || (k[i] == k[j] && ix[i] > ix[j]);
-    if (sw) {
-        float a = k[i]; k[i] = k[j]; k[j] = a;
-        uint32_t b = ix[i]; ix[i] = ix[j]; ix[j] = b;
-        uint32_t c = e[i]; e[i] = e[j]; e[j] = c;
-        float d = tn[i]; tn[i] = tn[j]; tn[j] = d;
-    }
+__device__ __forceinline__ void cswap64(uint64_t *k, int i, int j) {
+    const uint64_t a = k[i], b = k[j];
+    const bool sw = a > b;
+    k[i] = sw ? b : a;
+    k[j] = sw ? a : b;
 }
 
-template <int W> __device__ __forceinline__ void sort_children(float *k, uint32_t *ix, uint32_t *e, float *tn) {
+template <int W> __device__ __forceinline__ void sort_keys(uint64_t *k) {
     if constexpr (W == 2) {
-        cswap<W>(k, ix, e, tn, 0, 1);
+        cswap64(k, 0, 1);
     } else if constexpr (W == 4) {
-        cswap<W>(k, ix, e, tn, 0, 1); cswap<W>(k, ix, e, tn, 2, 3);
-        cswap<W>(k, ix, e, tn, 0, 2); cswap<W>(k, ix, e, tn, 1, 3);
-        cswap<W>(k, ix, e, tn, 1, 2);
+        cswap64(k, 0, 1); cswap64(k, 2, 3);
+        cswap64(k, 0, 2); cswap64(k, 1, 3);
+        cswap64(k, 1, 2);
     } else {  // Batcher odd-even merge sort, 19 comparators
-        cswap<W>(k, ix, e, tn, 0, 1); cswap<W>(k, ix, e, tn, 2, 3); cswap<W>(k, ix, e, tn, 4, 5); cswap<W>(k, ix, e, tn, 6, 7);
-        cswap<W>(k, ix, e, tn, 0, 2); cswap<W>(k, ix, e, tn, 1, 3); cswap<W>(k, ix, e, tn, 4, 6); cswap<W>(k, ix, e, tn, 5, 7);
-        cswap<W>(k, ix, e, tn, 1, 2); cswap<W>(k, ix, e, tn, 5, 6);
-        cswap<W>(k, ix, e, tn, 0, 4); cswap<W>(k, ix, e, tn, 1, 5); cswap<W>(k, ix, e, tn, 2, 6); cswap<W>(k, ix, e, tn, 3, 7);
-        cswap<W>(k, ix, e, tn, 2, 4); cswap<W>(k, ix, e, tn, 3, 5);
-        cswap<W>(k, ix, e, tn, 1, 2); cswap<W>(k, ix, e, tn, 3, 4); cswap<W>(k, ix, e, tn, 5, 6);
+        cswap64(k, 0, 1); cswap64(k, 2, 3); cswap64(k, 4, 5); cswap64(k, 6, 7);
+        cswap64(k, 0, 2); cswap64(k, 1, 3); cswap64(k, 4, 6); cswap64(k, 5, 7);
+        cswap64(k, 1, 2); cswap64(k, 5, 6);
+        cswap64(k, 0, 4); cswap64(k, 1, 5); cswap64(k, 2, 6); cswap64(k, 3, 7);
+        cswap64(k, 2, 4); cswap64(k, 3, 5);
+        cswap64(k, 1, 2); cswap64(k, 3, 4); cswap64(k, 5, 6);
     }
 }
 
-// One traversal step.  ANY: stop at the first hit.
+// Process the current wide node.  Precondition: at_node(t).
+//   closest: hit children in increasing (ord_key(tn), entry); ANY: in child order.
 template <int W, int S, bool ANY>
-__device__ __forceinline__ void trav_step(Trav &t, const StackMem &m, const uint32_t *__restrict__ wide,
-                                          const LeafRecord *__restrict__ rec) {
-    if (t.have) {
-        if (t.cur_tn > t.bt * kRobust) {
-            t.have = false;  // culled by a closer hit found since it was pushed
-        } else if (t.cur & kLeafBit) {
-            const uint32_t first = leaf_first(t.cur), cnt = leaf_count(t.cur);
-            for (uint32_t s = first; s < first + cnt; ++s) {
-                t.ntri++;
-                const LeafRecord r = rec[s];
-                float th = 0.0f, a = 0.0f, b = 0.0f;
-                bool hit = is_sphere_record(r) ? sphere_test(r, t.o, t.d, t.tmin, t.bt, th)
-                                               : triangle_test(r, t.o, t.d, t.tmin, t.bt, th, a, b);
-                if (hit && (th < t.bt || (th == t.bt && s < t.bslot))) {
-                    t.bt = th;
-                    t.bslot = s;
-                    t.bu = a;
-                    t.bv = b;
-                    if (ANY) {
-                        t.top = 0;
-                        t.nspill = 0;
-                        break;
-                    }
-                }
-            }
+__device__ __forceinline__ void node_step(Trav &t, const StackMem &m, const uint32_t *__restrict__ wide) {
+    if (t.cur_tn > t.bt * kRobust) {
+        t.have = false;  // culled by a closer hit found since it was pushed
+    } else {
+        WideNode<W> nd;
+        load_node<W>(wide, t.cur, nd);
+        uint64_t key[W];
+        int nh = 0;
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            const bool valid = nd.child[c] != kEmpty;
+            t.nbox += valid ? 1u : 0u;
+            float x0 = (nd.lo[0][c] - t.o.x) * t.idir.x, x1 = (nd.hi[0][c] - t.o.x) * t.idir.x;
+            float y0 = (nd.lo[1][c] - t.o.y) * t.idir.y, y1 = (nd.hi[1][c] - t.o.y) * t.idir.y;
+            float z0 = (nd.lo[2][c] - t.o.z) * t.idir.z, z1 = (nd.hi[2][c] - t.o.z) * t.idir.z;
+            float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), t.tmin));
+            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t.bt));
+            const bool hit = valid && tn <= tf * kRobust;
+            nh += hit ? 1 : 0;
+            const uint32_t hi = ANY ? (uint32_t)c : ord_key(tn);
+            key[c] = hit ? (((uint64_t)hi << 32) | nd.child[c]) : ~0ull;
+        }
+        sort_keys<W>(key);
+        if (nh == 0) {
             t.have = false;
         } else {
-            WideNode<W> nd;
-            load_node<W>(wide, t.cur, nd);
-            float key[W], tnv[W];
-            uint32_t ix[W], ent[W];
-            int nh = 0;
-            const float inf = __uint_as_float(0x7F800000u);
 #pragma unroll
-            for (int c = 0; c < W; ++c) {
-                const bool valid = nd.child[c] != kEmpty;
-                t.nbox += valid ? 1u : 0u;
-                float x0 = (nd.lo[0][c] - t.o.x) * t.idir.x, x1 = (nd.hi[0][c] - t.o.x) * t.idir.x;
-                float y0 = (nd.lo[1][c] - t.o.y) * t.idir.y, y1 = (nd.hi[1][c] - t.o.y) * t.idir.y;
-                float z0 = (nd.lo[2][c] - t.o.z) * t.idir.z, z1 = (nd.hi[2][c] - t.o.z) * t.idir.z;
-                float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), t.tmin));
-                float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t.bt));
-                const bool hit = valid && tn <= tf * kRobust;
-                nh += hit ? 1 : 0;
-                key[c] = hit ? (ANY ? 0.0f : tn) : inf;
-                tnv[c] = tn;
-                ix[c] = (uint32_t)c;
-                ent[c] = nd.child[c];
-            }
-            sort_children<W>(key, ix, ent, tnv);
-            if (nh == 0) {
-                t.have = false;
-            } else {
-#pragma unroll
-                for (int k = W - 1; k >= 1; --k)
-                    if (k < nh) push<S>(t, m, ent[k], tnv[k]);
-                t.cur = ent[0];
-                t.cur_tn = tnv[0];
-            }
+            for (int k = W - 1; k >= 1; --k)
+                if (k < nh) push<S>(t, m, (uint32_t)key[k], ANY ? kOrdNegInf : (uint32_t)(key[k] >> 32));
+            t.cur = (uint32_t)key[0];
+            t.cur_tn = ANY ? -__uint_as_float(0x7F800000u) : ord_dec((uint32_t)(key[0] >> 32));
         }
     }
     if (!t.have && t.top > 0) pop<S>(t, m);
+}
+
+// Test the primitives of the current leaf.  Precondition: at_leaf(t).
+template <int S, bool ANY>
+__device__ __forceinline__ void leaf_step(Trav &t, const StackMem &m, const LeafRecord *__restrict__ rec) {
+    if (t.cur_tn <= t.bt * kRobust) {
+        const uint32_t first = leaf_first(t.cur), cnt = leaf_count(t.cur);
+        for (uint32_t s = first; s < first + cnt; ++s) {
+            t.ntri++;
+            const LeafRecord r = rec[s];
+            float th = 0.0f, a = 0.0f, b = 0.0f;
+            bool hit = is_sphere_record(r) ? sphere_test(r, t.o, t.d, t.tmin, t.bt, th)
+                                           : triangle_test(r, t.o, t.d, t.tmin, t.bt, th, a, b);
+            if (hit && (th < t.bt || (th == t.bt && s < t.bslot))) {
+                t.bt = th;
+                t.bslot = s;
+                t.bu = a;
+                t.bv = b;
+                if (ANY) {
+                    t.top = 0;
+                    t.nspill = 0;
+                    break;
+                }
+            }
+        }
+    }
+    t.have = false;
+    if (t.top > 0) pop<S>(t, m);
+}
+
+// Wave-level choice of which lanes advance this iteration: leaves when enough lanes
+// wait on a leaf (or no lane is on a node), nodes otherwise.
+__device__ __forceinline__ bool choose_leaf_phase(bool on_node, bool on_leaf, uint32_t leaf_weight) {
+    const unsigned long long mn = __ballot(on_node), ml = __ballot(on_leaf);
+    if (ml == 0ull) return false;
+    if (mn == 0ull) return true;
+    return (uint32_t)__popcll(ml) * leaf_weight >= (uint32_t)__popcll(mn);
+}
+
+// Run one ray to completion (used where lanes are not refilled).
+template <int W, int S, bool ANY>
+__device__ __forceinline__ void trav_run(Trav &t, const StackMem &m, const uint32_t *wide, const LeafRecord *rec,
+                                         uint32_t leaf_weight) {
+    for (;;) {
+        const bool on_node = at_node(t), on_leaf = at_leaf(t);
+        if (__ballot(on_node || on_leaf) == 0ull) break;
+        if (choose_leaf_phase(on_node, on_leaf, leaf_weight)) {
+            if (on_leaf) leaf_step<S, ANY>(t, m, rec);
+        } else {
+            if (on_node) node_step<W, S, ANY>(t, m, wide);
+        }
+    }
 }
 
 }  // namespace rtgpu

@@ -80,6 +80,7 @@ _SIGS = {
     "rt_util_free": (_i32, [_vp]),
     "rt_util_memcpy_htod": (_i32, [_vp, _vp, _sz]),
     "rt_util_memcpy_dtoh": (_i32, [_vp, _vp, _sz]),
+    "rt_util_memcpy_dtod": (_i32, [_vp, _vp, _sz]),
     "rt_util_memset": (_i32, [_vp, _i32, _sz]),
     "rt_util_synchronize": (_i32, []),
     "rt_util_stream_create": (_i32, [C.POINTER(_vp)]),
@@ -134,6 +135,10 @@ def device_name(dev: int = 0) -> str:
     buf = C.create_string_buffer(256)
     _check(lib.rt_util_device_name(dev, buf, 256), what="device name")
     return buf.value.decode()
+
+
+def copy_dtod(dst: "DeviceArray", src: "DeviceArray", nbytes: int):
+    _check(lib.rt_util_memcpy_dtod(dst.ptr, src.ptr, nbytes), what="memcpy dtod")
 
 
 def synchronize():

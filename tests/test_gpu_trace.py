@@ -34,7 +34,7 @@ def test_closest_hit_clusters(ctx, native, O, width, leaf):
     assert np.array_equal(gc[:, 0], oc["n_box"]) and np.array_equal(gc[:, 1], oc["n_tri"])
     ob = osc.trace(rays[:4000], O.BRUTE)
     assert compare_hits(g[:4000], ob) == 0
-    assert np.mean(g["prim"] != L.NO_HIT) > 0.05
+    assert np.mean(g["prim"] != L.NO_HIT) > 0.01
     ctx.release(sid)
 
 
