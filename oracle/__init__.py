@@ -160,8 +160,8 @@ def aabb_hit_ref(o, d, bmin, bmax) -> bool:
 
 def transform_mat4(translation, scale, rotation) -> np.ndarray:
     out = np.zeros(16, np.float32)
-    lib.orc_transform_mat4(_f3(translation).ctypes.data, _f3(scale).ctypes.data, _f3(rotation).ctypes.data,
-                           out.ctypes.data)
+    tr, sc, rot = _f3(translation), _f3(scale), _f3(rotation)  # keep the arrays alive across the call
+    lib.orc_transform_mat4(tr.ctypes.data, sc.ctypes.data, rot.ctypes.data, out.ctypes.data)
     return out
 
 
@@ -185,8 +185,9 @@ def morton_codes(tris, spheres, emin, emax, flags=0):
     tris = np.ascontiguousarray(tris, dtype=TRIANGLE)
     spheres = np.ascontiguousarray(spheres, dtype=SPHERE)
     out = np.zeros(len(tris) + len(spheres), np.uint32)
-    lib.orc_morton_codes(_p(tris), len(tris), _p(spheres), len(spheres), _f3(emin).ctypes.data,
-                         _f3(emax).ctypes.data, flags, out.ctypes.data)
+    mn, mx = _f3(emin), _f3(emax)
+    lib.orc_morton_codes(_p(tris), len(tris), _p(spheres), len(spheres), mn.ctypes.data, mx.ctypes.data, flags,
+                         out.ctypes.data)
     return out
 
 
