@@ -82,6 +82,74 @@ def copy_calibration(mib: int = 1024, reps: int = 5):
     print(json.dumps({"copy_calibration_bytes_per_dispatch": n, "reps": reps}), flush=True)
 
 
+def pathtrace_bench(ctx, N, W, L, frames: int, cpu_seconds: float, threads: int):
+    """Path tracing: noise-displaced icosphere (1,003,520 triangles; 60/30/10% lambertian/metal/dielectric)
+    + emissive quad, 1920x1080, 16 spp, depth 8; one frame = rt_accum_clear + rt_render."""
+    t0 = time.time()
+    s = W.icosphere_scene(freq=224, width=1920, height=1080, spp=16, depth=8)
+    sid = ctx.upload(s["tris"], None, s["materials"])
+    w, h, spp = s["width"], s["height"], s["spp"]
+    npix = w * h
+    log(f"pathtrace scene: {len(s['tris'])} triangles, setup {time.time() - t0:.1f} s")
+    acc = N.DeviceArray(16 * npix)
+    segs = N.DeviceArray(8)
+    stream = N.Stream()
+    best = None
+    for cfg in [(128, 8, 1, 16), (128, 16, 1, 16), (256, 8, 1, 32), (64, 16, 1, 16), (256, 16, 1, 32)]:
+        c = N.launch_config(*cfg)
+        ctx.accum_clear(acc, npix, stream=stream)
+        ctx.render(sid, s["params"], w, h, spp, acc, cfg=c, stream=stream)
+        e0, e1 = N.Event(), N.Event()
+        e0.record(stream)
+        ctx.accum_clear(acc, npix, stream=stream)
+        ctx.render(sid, s["params"], w, h, spp, acc, cfg=c, stream=stream)
+        e1.record(stream)
+        ms = e0.elapsed_ms(e1)
+        if best is None or ms < best[1]:
+            best = (cfg, ms)
+    cfg = N.launch_config(*best[0])
+    ev = [N.Event() for _ in range(frames + 1)]
+    segs.zero()
+    ev[0].record(stream)
+    for k in range(frames):
+        ctx.accum_clear(acc, npix, stream=stream)
+        ctx.render(sid, s["params"], w, h, spp, acc, segments=segs, cfg=cfg, stream=stream)
+        ev[k + 1].record(stream)
+    stream.synchronize()
+    total_ms = ev[0].elapsed_ms(ev[-1])
+    ms = total_ms / frames
+    seg = int(segs.to_numpy(np.uint64, 1)[0]) // frames
+    seg_all = seg + npix * (spp - 1)  # primary segments reused for samples 2..spp
+    out = {"workload": f"path trace: {len(s['tris'])} triangles (icosphere freq 224, noise-displaced, per-triangle "
+                       f"lambertian/metal/dielectric) + emissive quad, {w}x{h}, {spp} spp, depth 8",
+           "ms_per_frame": round(ms, 3), "segments_traced_per_frame": seg,
+           "segments_per_frame_incl_reused_primaries": seg_all,
+           "msegments_per_s": round(seg / (ms * 1e-3) / 1e6, 1),
+           "msamples_per_s": round(npix * spp / (ms * 1e-3) / 1e6, 1),
+           "launch": dict(zip(("block", "lds_stack", "persistent", "refill"), best[0]))}
+    log(f"pathtrace: {ms:.2f} ms/frame, {out['msegments_per_s']} Msegments/s")
+    if cpu_seconds > 0:
+        import oracle as O
+        nodes, _, slots, _, _ = O.build_lbvh(s["tris"])
+        osc = O.OracleScene(s["tris"], None, s["materials"], nodes, slots)
+        rows, spent, segc, y = 0, 0.0, 0, h // 2
+        accum = np.zeros((npix, 4), np.float32)
+        accum[:, 3] = 1.0
+        while spent < cpu_seconds and y < h:
+            t1 = time.perf_counter()
+            _, sg = osc.render(s["params"], w, h, spp, accum=accum, method=O.REF_BVH, rows=(y, y + 1),
+                               threads=threads)
+            spent += time.perf_counter() - t1
+            segc += sg
+            rows += 1
+            y += 1
+        out["cpu_baseline"] = {"value": round(segc / spent / 1e6, 3), "unit": "Msegments/s", "cores": threads,
+                               "kind": "port", "sample": f"{rows} image rows ({rows * w} pixels x {spp} spp) of the "
+                               f"same frame from row {h // 2}, oracle path tracer on its reference LBVH traversal"}
+    ctx.release(sid)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +166,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true", help="only run the PMC calibration copies")
+    ap.add_argument("--no-pathtrace", action="store_true", help="skip the path-tracing measurement")
+    ap.add_argument("--pt-frames", type=int, default=5)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -217,7 +287,8 @@ def main():
                                                         "persistent": tune.cfg.persistent,
                                                         "refill": tune.cfg.refill},
                    "device": dev},
-        "achieved_hbm_gbs": round(achieved, 1),
+        "algorithmic_gbs": round(achieved, 1),
+        "hbm_traffic_gbs": None if traffic is None else round(traffic / (avg_kernel_ms * 1e-3) / 1e9, 1),
         "mean_bytes_per_ray": round(mean_b, 1),
         "mean_aabb_tests": round(float(counts[:, 0].mean()), 2),
         "mean_tri_tests": round(float(counts[:, 1].mean()), 2),
@@ -227,8 +298,16 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    if fixed is not None:
+        out["config"]["launch"] = dict(zip(("block", "lds_stack", "persistent", "refill"),
+                                           (fixed.block, fixed.lds_stack, fixed.persistent, fixed.refill)))
     ctx.release(sid)
+    del d_rays, d_hits, d_counts
+    if not args.no_pathtrace:
+        threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
+        out["pathtrace"] = pathtrace_bench(ctx, N, W, L, args.pt_frames,
+                                           0.0 if args.no_cpu_baseline else args.cpu_seconds / 2, threads)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

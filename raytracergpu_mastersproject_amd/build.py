@@ -21,7 +21,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "librtgpu.so"
 GEN_SRC = HOST / "workloads_gen.cpp"
 GEN_LIB = PKG / "libworkloads.so"
-TOOLS = {"rtgpu_render": HOST / "main.cpp"}
+TOOLS = {"rtgpu_render": HOST / "main.cpp", "rtgpu_scene_dump": HOST / "scene_dump.cpp"}
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RTGPU_ARCH", "gfx950")
@@ -48,7 +48,8 @@ def _run(cmd: list[str]) -> None:
 
 def sources() -> tuple[list[Path], list[Path]]:
     hip = sorted(CSRC.glob("*.hip"))
-    cpp = sorted(p for p in HOST.glob("*.cpp") if p.name not in ("main.cpp", GEN_SRC.name))
+    tool_srcs = {p.name for p in TOOLS.values()}
+    cpp = sorted(p for p in HOST.glob("*.cpp") if p.name not in tool_srcs and p.name != GEN_SRC.name)
     return hip, cpp
 
 
