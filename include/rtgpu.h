@@ -181,6 +181,23 @@ int rt_render(rt_context *ctx, uint32_t scene_id, const rt_params *params, uint3
 int rt_resolve(rt_context *ctx, const float *accum, uint32_t npix, uint32_t spp, float *out_rgba,
                void *stream);
 
+/* One image of a batched render: the arguments of one rt_render call. */
+typedef struct rt_render_job {
+    uint32_t scene_id;
+    uint32_t width, height, spp;
+    uint32_t flags;     /* RT_RENDER_* */
+    uint32_t _reserved;
+    float *accum;       /* device RGBA32F image, width*height*16 bytes */
+    rt_params params;   /* this image's UBO */
+} rt_render_job;
+
+/* Renders every job's image in one persistent launch per BVH width present (no such
+ * call exists in the reference, whose S2 renders one image: RaytracerBVH.cpp:1000-1078).
+ * Each image is bit-identical to rt_render of its job alone.  jobs: host array; the
+ * accum images must not overlap.  segments (device, may be NULL) as for rt_render. */
+int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, unsigned long long *segments,
+                    const rt_launch_config *cfg, void *stream);
+
 /* ---------------- batched persistent launch (many scenes, one launch) ---------------- */
 
 typedef struct rt_batch_item {
@@ -208,6 +225,89 @@ typedef struct rt_tune_result {
  * for that scene.  rays/hits: device; the hits buffer is overwritten. */
 int rt_autotune(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n, rt_hit *hits,
                 uint32_t flags, rt_tune_result *out);
+
+/* ---------------- serving loop ----------------
+ * No counterpart in the reference, which renders one scene per frame on one thread
+ * (RaytracerBVH.hpp:517-571).  A server keeps scenes resident (uploaded to its context),
+ * queues render and ray-query requests, admits them in arrival order while a slot and
+ * enough of its device arena are free, and advances every admitted request in
+ * scheduling steps: each step is one fused launch over slices of all admitted requests
+ * (samples of renders, rays of queries), so requests that arrive while others run share
+ * the device with them.  A request's result does not depend on what it was batched
+ * with.  One worker thread per server drives the device; the context's other entry
+ * points may be called meanwhile from other threads, but scenes that requests use must
+ * not be released until those requests finish (rt_scene_release then reports
+ * RT_ERR_BUSY), and servers must be destroyed before their context. */
+typedef struct rt_server rt_server;
+
+#define RT_REQ_RENDER 0u
+#define RT_REQ_TRACE 1u
+#define RT_REQ_OUT_ACCUM 0x100u /* render_flags: return the raw accumulation instead of the resolved image */
+
+#define RT_SERVER_SERIAL 1u /* one request per step, whole, through the single-scene kernels (the unbatched baseline) */
+
+typedef struct rt_server_config {
+    uint64_t arena_bytes;        /* device memory for request buffers: the admission budget (0 = 4 GiB) */
+    uint32_t max_active;         /* requests admitted at once (0 = 64) */
+    uint32_t step_us;            /* target GPU time of one scheduling step (0 = 4000) */
+    uint32_t pipeline;           /* steps enqueued ahead of the device (0 = 2) */
+    uint32_t flags;              /* RT_SERVER_* */
+    rt_launch_config render_cfg; /* zeros = defaults */
+    rt_launch_config trace_cfg;
+} rt_server_config;
+
+typedef struct rt_request {
+    uint32_t kind;         /* RT_REQ_RENDER or RT_REQ_TRACE */
+    uint32_t scene_id;
+    /* render: the arguments of rt_render (accumulation cleared to (0,0,0,1) first) */
+    uint32_t width, height, spp;
+    uint32_t render_flags; /* RT_RENDER_* | RT_REQ_OUT_ACCUM */
+    rt_params params;
+    /* ray query */
+    uint32_t nrays;
+    uint32_t trace_flags;  /* RT_TRACE_ANY */
+    const rt_ray *rays;    /* host memory, copied at admission */
+    /* host memory receiving the result when the request completes (NULL = discard):
+     * render: width*height RGBA32F; ray query: nrays rt_hit */
+    void *out;
+} rt_request;
+
+typedef struct rt_request_stats {
+    uint64_t id;
+    double submit_ms, admit_ms, done_ms; /* host clock, ms since the server was created */
+    uint64_t device_bytes;               /* arena bytes the request held */
+    uint32_t steps;                      /* scheduling steps it took part in */
+    int32_t status;                      /* RT_OK or the error that failed it */
+} rt_request_stats;
+
+typedef struct rt_server_stats {
+    uint64_t submitted, completed, failed, steps;
+    uint64_t samples;      /* pixel-samples rendered */
+    uint64_t segments;     /* path segments traced by renders */
+    uint64_t rays;         /* rays traced by ray queries */
+    uint64_t arena_bytes, arena_peak;
+    double busy_ms;        /* device time of all steps */
+    double wall_ms;        /* since the server was created */
+    uint32_t active, queued;
+} rt_server_stats;
+
+int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **out);
+/* Finishes every submitted request, then frees the server. */
+int rt_server_destroy(rt_server *srv);
+/* Queues a request (RT_ERR_BUDGET if it alone exceeds the arena).  Host arrays are read
+ * at admission and `out` is written at completion, so both must stay valid until then. */
+int rt_server_submit(rt_server *srv, const rt_request *req, uint64_t *id);
+/* Waits for a request (timeout_ms < 0: no limit; RT_ERR_BUSY on timeout).  Returns the
+ * request's own status.  stats may be NULL. */
+int rt_server_wait(rt_server *srv, uint64_t id, int timeout_ms, rt_request_stats *stats);
+/* Waits until every submitted request has finished. */
+int rt_server_drain(rt_server *srv);
+int rt_server_stats_get(rt_server *srv, rt_server_stats *out);
+/* Open-loop replay: submits reqs[i] at arrival_ms[i] (ascending) after the call starts,
+ * waits for all of them and writes their stats (n entries; times relative to the call). */
+int rt_server_replay(rt_server *srv, const rt_request *reqs, const double *arrival_ms, uint32_t n,
+                     rt_request_stats *out);
+const char *rt_server_last_error(const rt_server *srv);
 
 /* ---------------- device plumbing for hosts without their own HIP runtime ---------------- */
 int rt_util_malloc(void **ptr, size_t bytes);

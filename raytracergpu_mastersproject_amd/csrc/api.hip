@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "rt_internal.hpp"
@@ -30,6 +31,8 @@ static_assert(sizeof(LeafRecord) == 64, "leaf record");
 namespace {
 
 template <class F> int guarded(rt_context *ctx, F &&f) {
+    std::unique_lock<std::recursive_mutex> lk;
+    if (ctx) lk = std::unique_lock<std::recursive_mutex>(ctx->mu);
     try {
         if (ctx) {
             hipError_t e = hipSetDevice(ctx->device);
@@ -113,6 +116,9 @@ int rt_scene_release(rt_context *ctx, uint32_t scene_id) {
     return guarded(ctx, [&] {
         auto it = ctx->scenes.find(scene_id);
         if (it == ctx->scenes.end()) throw Error(RT_ERR_NOT_FOUND, "unknown scene id " + std::to_string(scene_id));
+        if (it->second->users)
+            throw Error(RT_ERR_BUSY, "scene " + std::to_string(scene_id) + " is used by " +
+                                         std::to_string(it->second->users) + " server request(s)");
         RT_HIP(hipDeviceSynchronize());  // in-flight work may still read the scene
         ctx->scenes.erase(it);
     });
@@ -193,6 +199,33 @@ int rt_render(rt_context *ctx, uint32_t scene_id, const rt_params *params, uint3
 int rt_resolve(rt_context *ctx, const float *accum, uint32_t npix, uint32_t spp, float *out_rgba, void *stream) {
     if (!ctx) return RT_ERR_INVALID;
     return guarded(ctx, [&] { launch_resolve(accum, npix, spp, out_rgba, static_cast<hipStream_t>(stream)); });
+}
+
+int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, unsigned long long *segments,
+                    const rt_launch_config *cfg, void *stream) {
+    if (!ctx || (njobs && !jobs)) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        if (njobs == 0) return;
+        const LaunchCfg c = resolve_cfg(ctx, ctx->scene(jobs[0].scene_id), cfg, false);
+        std::map<uint32_t, std::vector<RenderJob>> by_width;
+        std::vector<std::pair<uintptr_t, uintptr_t>> ranges;
+        for (uint32_t i = 0; i < njobs; ++i) {
+            const rt_render_job &j = jobs[i];
+            RT_REQUIRE((j.flags & ~(RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE)) == 0,
+                       "unknown render flags");
+            const Scene &s = ctx->scene(j.scene_id);
+            check_stack(s, c);
+            RenderJob rj = make_render_job(s, j.params, j.width, j.height, j.spp, j.flags, j.accum);
+            const uintptr_t b = reinterpret_cast<uintptr_t>(j.accum);
+            ranges.emplace_back(b, b + 16ull * j.width * j.height);
+            by_width[s.width].push_back(rj);
+        }
+        std::sort(ranges.begin(), ranges.end());
+        for (size_t i = 1; i < ranges.size(); ++i)
+            RT_REQUIRE(ranges[i].first >= ranges[i - 1].second, "rt_render_batch: accumulation images overlap");
+        for (auto &kv : by_width)
+            launch_render_jobs(ctx, kv.second, kv.first, segments, c, static_cast<hipStream_t>(stream));
+    });
 }
 
 int rt_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems, uint32_t flags,

@@ -14,6 +14,14 @@
 // first segment of every sample of a pixel is the same ray with the same
 // closest hit; it is traced once per pixel and reused (RT_RENDER_NO_PRIMARY_CACHE
 // disables this).
+//
+// One launch may render several images (jobs), each with its own scene, camera,
+// image and sample count: the work items of all jobs are concatenated, each job
+// starting on a chunk boundary, so a wave's chunk belongs to one job and is found
+// with one wave-uniform search of the job table.  Since a pixel's samples depend only
+// on its accumulator (the alpha carry), rendering spp = a and then spp = b into the
+// same image equals rendering spp = a + b, which lets a scheduler hand out samples
+// in slices.
 #include <algorithm>
 #include <cmath>
 
@@ -52,24 +60,18 @@ void camera_setup(const rt_params &p, uint32_t width, uint32_t height, float out
 namespace {
 
 struct RenderArgs {
-    const uint32_t *wide;
-    const LeafRecord *rec;
-    const uint32_t *slot_mat;
-    const rt_material *mats;
-    uint32_t nwide;
-    uint32_t width, height, spp, depth, random_state, flags, refill, leaf_weight;
-    uint32_t tiles_x, nitems;
-    float cam[12];
-    float *accum;
+    RenderJob job0;         // single-image launches
+    const RenderJob *jobs;  // multi-image launches (device table)
+    uint32_t njobs, nitems, refill, leaf_weight;
     unsigned long long *segments;
     unsigned int *work;
 };
 
 enum : uint32_t { kIdle = 0, kTracing = 1, kResolved = 2 };
-constexpr uint32_t kRenderChunk = 128;  // pixels (two 8x8 tiles) per atomic on the work counter
 
 struct PixelState {
     uint32_t phase;
+    uint32_t job;  // index into the job table (multi-image launches)
     uint32_t pix;  // y*width + x
     uint32_t x, y;
     uint32_t seed0, st;
@@ -82,16 +84,34 @@ struct PixelState {
     uint32_t pc_slot;  // cached first segment
     float pc_t;
     bool pc_valid;
+    const uint32_t *wide;  // the job's BVH (multi-image launches)
+    const LeafRecord *rec;
 };
 
-__device__ __forceinline__ void primary(const RenderArgs &a, uint32_t x, uint32_t y, v3 &o, v3 &d) {
-    v3 p00{a.cam[0], a.cam[1], a.cam[2]}, du{a.cam[3], a.cam[4], a.cam[5]};
-    v3 dv{a.cam[6], a.cam[7], a.cam[8]}, org{a.cam[9], a.cam[10], a.cam[11]};
+template <bool MULTI> __device__ __forceinline__ const RenderJob &job_of(const RenderArgs &a, const PixelState &ps) {
+    if constexpr (MULTI) return a.jobs[ps.job];
+    else return a.job0;
+}
+
+template <bool MULTI> __device__ __forceinline__ const LeafRecord *rec_of(const RenderArgs &a, const PixelState &ps) {
+    if constexpr (MULTI) return ps.rec;
+    else return a.job0.rec;
+}
+
+template <bool MULTI> __device__ __forceinline__ const uint32_t *wide_of(const RenderArgs &a, const PixelState &ps) {
+    if constexpr (MULTI) return ps.wide;
+    else return a.job0.wide;
+}
+
+__device__ __forceinline__ void primary(const RenderJob &J, uint32_t x, uint32_t y, v3 &o, v3 &d) {
+    v3 p00{J.cam[0], J.cam[1], J.cam[2]}, du{J.cam[3], J.cam[4], J.cam[5]};
+    v3 dv{J.cam[6], J.cam[7], J.cam[8]}, org{J.cam[9], J.cam[10], J.cam[11]};
     v3 ps = (p00 + (float)x * du) + (float)y * dv;
     o = org;
     d = normalize(normalize(ps - org));  // getRay normalises; rayColor normalises again
 }
 
+template <bool MULTI>
 __device__ __forceinline__ void begin_segment(const RenderArgs &a, PixelState &ps, Trav &t, uint32_t &segs) {
     if (ps.bounce == 0 && ps.pc_valid) {
         ps.hslot = ps.pc_slot;
@@ -99,45 +119,51 @@ __device__ __forceinline__ void begin_segment(const RenderArgs &a, PixelState &p
         ps.phase = kResolved;
         return;
     }
-    trav_init(t, ps.o, ps.d, 0.001f, 10000000.0f, a.nwide);
+    trav_init(t, ps.o, ps.d, 0.001f, 10000000.0f, job_of<MULTI>(a, ps).nwide);
     ps.phase = kTracing;
     segs++;
 }
 
+template <bool MULTI>
 __device__ __forceinline__ void begin_sample(const RenderArgs &a, PixelState &ps, Trav &t, uint32_t &segs) {
+    const RenderJob &J = job_of<MULTI>(a, ps);
     ps.st = ps.seed0 + f2u_sat(ps.acc[3] * 4294967294.0f);  // the literal is 2^32 in fp32
     ps.next = rng_next(ps.st);
-    primary(a, ps.x, ps.y, ps.o, ps.d);
+    primary(J, ps.x, ps.y, ps.o, ps.d);
     ps.color = v3{0.0f, 0.0f, 0.0f};
     ps.atten = v3{1.0f, 1.0f, 1.0f};
     ps.bounce = 0;
-    if (a.depth == 0) {
+    if (J.depth == 0) {
         ps.hslot = kEmpty;
         ps.phase = kResolved;
         return;
     }
-    begin_segment(a, ps, t, segs);
+    begin_segment<MULTI>(a, ps, t, segs);
 }
 
 // returns false when the pixel is finished
+template <bool MULTI>
 __device__ __forceinline__ bool end_sample(const RenderArgs &a, PixelState &ps, Trav &t, uint32_t &segs) {
+    const RenderJob &J = job_of<MULTI>(a, ps);
     ps.acc[0] = ps.color.x + ps.acc[0];
     ps.acc[1] = ps.color.y + ps.acc[1];
     ps.acc[2] = ps.color.z + ps.acc[2];
     ps.acc[3] = ps.next;
     ps.sample++;
-    if (ps.sample < a.spp) {
-        begin_sample(a, ps, t, segs);
+    if (ps.sample < J.spp) {
+        begin_sample<MULTI>(a, ps, t, segs);
         return true;
     }
-    reinterpret_cast<float4 *>(a.accum)[ps.pix] = make_float4(ps.acc[0], ps.acc[1], ps.acc[2], ps.acc[3]);
+    reinterpret_cast<float4 *>(J.accum)[ps.pix] = make_float4(ps.acc[0], ps.acc[1], ps.acc[2], ps.acc[3]);
     ps.phase = kIdle;
     return false;
 }
 
 // emitted + scatter for a resolved segment (raytraceBVH.comp:94-115, 276-308)
+template <bool MULTI>
 __device__ __forceinline__ void shade(const RenderArgs &a, PixelState &ps, Trav &t, uint32_t &segs) {
-    if (ps.bounce == 0 && !ps.pc_valid && !(a.flags & RT_RENDER_NO_PRIMARY_CACHE)) {
+    const RenderJob &J = job_of<MULTI>(a, ps);
+    if (ps.bounce == 0 && !ps.pc_valid && !(J.flags & RT_RENDER_NO_PRIMARY_CACHE)) {
         ps.pc_slot = ps.hslot;
         ps.pc_t = ps.ht;
         ps.pc_valid = true;
@@ -145,11 +171,11 @@ __device__ __forceinline__ void shade(const RenderArgs &a, PixelState &ps, Trav 
     if (ps.hslot == kEmpty) {
         const v3 background{0.0f, 0.0f, 0.0f};  // _BACKGROUND_COLOR
         ps.color = ps.color + mul(background, ps.atten);
-        end_sample(a, ps, t, segs);
+        end_sample<MULTI>(a, ps, t, segs);
         return;
     }
-    const LeafRecord r = a.rec[ps.hslot];
-    const rt_material m = a.mats[a.slot_mat[ps.hslot]];
+    const LeafRecord r = rec_of<MULTI>(a, ps)[ps.hslot];
+    const rt_material m = J.mats[J.slot_mat[ps.hslot]];
     const v3 alb{m.albedo[0], m.albedo[1], m.albedo[2]};
     const v3 p = ps.o + ps.ht * ps.d;
     v3 n;
@@ -161,7 +187,7 @@ __device__ __forceinline__ void shade(const RenderArgs &a, PixelState &ps, Trav 
     ps.color = ps.color + mul(emitted, ps.atten);
     bool scattered = false;
     v3 att{0.0f, 0.0f, 0.0f}, nd{0.0f, 0.0f, 0.0f};
-    const bool ref_mats = (a.flags & RT_RENDER_REFERENCE_MATERIALS) != 0;
+    const bool ref_mats = (J.flags & RT_RENDER_REFERENCE_MATERIALS) != 0;
     if (m.materialType == RT_DIFFUSE) {
         att = alb;
         v3 ruv = normalize(random_in_unit_sphere(ps.st));
@@ -186,28 +212,40 @@ __device__ __forceinline__ void shade(const RenderArgs &a, PixelState &ps, Trav 
         scattered = true;
     }
     if (!scattered) {
-        end_sample(a, ps, t, segs);
+        end_sample<MULTI>(a, ps, t, segs);
         return;
     }
     ps.atten = mul(ps.atten, att);
     ps.o = p;
     ps.d = nd;
     ps.bounce++;
-    if (ps.bounce >= a.depth) {
-        end_sample(a, ps, t, segs);
+    if (ps.bounce >= J.depth) {
+        end_sample<MULTI>(a, ps, t, segs);
         return;
     }
-    begin_segment(a, ps, t, segs);
+    begin_segment<MULTI>(a, ps, t, segs);
 }
 
-__device__ __forceinline__ bool item_to_pixel(const RenderArgs &a, uint32_t item, uint32_t &x, uint32_t &y) {
-    const uint32_t tile = item >> 6, within = item & 63u;
-    x = (tile % a.tiles_x) * 8u + (within & 7u);
-    y = (tile / a.tiles_x) * 8u + (within >> 3);
-    return x < a.width && y < a.height;
+// `local` is the item index within its job
+__device__ __forceinline__ bool item_to_pixel(const RenderJob &J, uint32_t local, uint32_t &x, uint32_t &y) {
+    const uint32_t tile = local >> 6, within = local & 63u;
+    x = (tile % J.tiles_x) * 8u + (within & 7u);
+    y = (tile / J.tiles_x) * 8u + (within >> 3);
+    return x < J.width && y < J.height;
 }
 
-template <int W, int S>
+// last job whose first item is <= g (wave-uniform arguments, so this runs on scalar loads)
+__device__ __forceinline__ uint32_t find_job(const RenderJob *jobs, uint32_t njobs, uint32_t g) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].first_item <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int W, int S, bool MULTI>
 __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
     extern __shared__ uint2 lds[];
     uint2 spill[kSpillCap];
@@ -216,12 +254,13 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
     PixelState ps;
     ps.phase = kIdle;
+    ps.job = 0;
     Trav t;
     t.have = false;
     t.top = 0;
     uint32_t segs = 0;
     bool exhausted = false;
-    uint32_t q_next = 0, q_end = 0;  // the wave's current chunk of work items (wave-uniform)
+    uint32_t q_next = 0, q_end = 0, q_job = 0;  // the wave's current chunk of work items (wave-uniform)
     for (;;) {
         const unsigned long long idle = __ballot(ps.phase == kIdle);
         const uint32_t nidle = (uint32_t)__popcll(idle);
@@ -230,27 +269,38 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 uint32_t base = 0;
                 if ((int)lane == leader) base = atomicAdd(a.work, kRenderChunk);
-                base = (uint32_t)__shfl((int)base, leader);
-                if (base >= a.nitems) exhausted = true;
-                else { q_next = base; q_end = (uint32_t)min((uint64_t)base + kRenderChunk, (uint64_t)a.nitems); }
+                base = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)base, leader));
+                if (base >= a.nitems) {
+                    exhausted = true;
+                } else {
+                    q_next = base;
+                    q_end = (uint32_t)min((uint64_t)base + kRenderChunk, (uint64_t)a.nitems);
+                    if constexpr (MULTI) q_job = find_job(a.jobs, a.njobs, base);
+                }
             }
             const uint32_t take = exhausted ? 0u : min(nidle, q_end - q_next);
             const uint32_t r = (uint32_t)__popcll(idle & lt);
             const uint32_t item = q_next + r;
             q_next += take;
             if (ps.phase == kIdle && r < take) {
+                ps.job = q_job;
+                const RenderJob &J = job_of<MULTI>(a, ps);
                 uint32_t x, y;
-                if (item_to_pixel(a, item, x, y)) {
+                if (item_to_pixel(J, item - J.first_item, x, y)) {
+                    if constexpr (MULTI) {
+                        ps.wide = J.wide;
+                        ps.rec = J.rec;
+                    }
                     ps.x = x;
                     ps.y = y;
-                    ps.pix = y * a.width + x;
-                    const float4 acc = reinterpret_cast<const float4 *>(a.accum)[ps.pix];
+                    ps.pix = y * J.width + x;
+                    const float4 acc = reinterpret_cast<const float4 *>(J.accum)[ps.pix];
                     ps.acc[0] = acc.x; ps.acc[1] = acc.y; ps.acc[2] = acc.z; ps.acc[3] = acc.w;
-                    ps.seed0 = rng_seed(x, y, a.random_state);
+                    ps.seed0 = rng_seed(x, y, J.random_state);
                     ps.sample = 0;
                     ps.pc_valid = false;
-                    if (a.spp == 0) ps.phase = kIdle;
-                    else begin_sample(a, ps, t, segs);
+                    if (J.spp == 0) ps.phase = kIdle;
+                    else begin_sample<MULTI>(a, ps, t, segs);
                 }
             }
         }
@@ -258,9 +308,9 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
         const bool tracing = ps.phase == kTracing;
         const bool on_node = tracing && at_node(t), on_leaf = tracing && at_leaf(t);
         if (choose_leaf_phase(on_node, on_leaf, a.leaf_weight)) {
-            if (on_leaf) leaf_step<S, false>(t, m, a.rec);
+            if (on_leaf) leaf_step<S, false>(t, m, rec_of<MULTI>(a, ps));
         } else {
-            if (on_node) node_step<W, S, false>(t, m, a.wide);
+            if (on_node) node_step<W, S, false>(t, m, wide_of<MULTI>(a, ps));
         }
         if (tracing && trav_done(t)) {
             ps.hslot = t.bslot;
@@ -268,7 +318,7 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
             ps.phase = kResolved;
         }
         // resolved segments may chain (cached primary hit, depth reached): bounded by spp*depth
-        while (ps.phase == kResolved) shade(a, ps, t, segs);
+        while (ps.phase == kResolved) shade<MULTI>(a, ps, t, segs);
     }
     if (a.segments) {
         unsigned long long s = segs;
@@ -298,12 +348,35 @@ __global__ void k_zero(unsigned int *c) {
     if (threadIdx.x == 0) c[0] = 0u;
 }
 
-void *pick_render(uint32_t width, uint32_t s) {
+template <int W, bool MULTI> void *pick_render_s(uint32_t s) {
+    return s == 4 ? (void *)k_render<W, 4, MULTI> : s == 16 ? (void *)k_render<W, 16, MULTI> : (void *)k_render<W, 8, MULTI>;
+}
+
+void *pick_render(uint32_t width, uint32_t s, bool multi) {
     switch (width) {
-        case 2: return s == 4 ? (void *)k_render<2, 4> : s == 16 ? (void *)k_render<2, 16> : (void *)k_render<2, 8>;
-        case 8: return s == 4 ? (void *)k_render<8, 4> : s == 16 ? (void *)k_render<8, 16> : (void *)k_render<8, 8>;
-        default: return s == 4 ? (void *)k_render<4, 4> : s == 16 ? (void *)k_render<4, 16> : (void *)k_render<4, 8>;
+        case 2: return multi ? pick_render_s<2, true>(s) : pick_render_s<2, false>(s);
+        case 8: return multi ? pick_render_s<8, true>(s) : pick_render_s<8, false>(s);
+        default: return multi ? pick_render_s<4, true>(s) : pick_render_s<4, false>(s);
     }
+}
+
+void launch_render_kernel(rt_context *ctx, RenderArgs &a, uint32_t bvh_width, bool multi, const LaunchCfg &c,
+                          hipStream_t stream) {
+    a.refill = c.refill;
+    a.leaf_weight = c.leaf_weight;
+    a.work = ctx->counter_slot();
+    k_zero<<<1, 64, 0, stream>>>(a.work);
+    const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
+    void *k = pick_render(bvh_width, c.lds_stack, multi);
+    int per_cu = 0;
+    RT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)c.block, lds));
+    const uint64_t waves_needed = (a.nitems + kRenderChunk - 1) / kRenderChunk;  // no more waves than chunks
+    const uint64_t full = (uint64_t)std::max(1, per_cu) * (uint32_t)ctx->num_cus;
+    const uint64_t wpb = c.block / 64u;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min(full, (waves_needed + wpb - 1) / wpb));
+    void *args[] = {&a};
+    RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
+    RT_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -325,45 +398,66 @@ void launch_resolve(const float *accum, uint32_t npix, uint32_t spp, float *out,
     RT_HIP(hipGetLastError());
 }
 
-void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,
-                   uint32_t flags, float *accum, unsigned long long *segments, const LaunchCfg &c,
-                   hipStream_t stream) {
+RenderJob make_render_job(const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,
+                          uint32_t flags, float *accum) {
     RT_REQUIRE(accum, "accum is NULL");
     RT_REQUIRE(width > 0 && height > 0, "image size must be positive");
     RT_REQUIRE((uint64_t)width * height < (1ull << 31), "image too large");
+    RenderJob j{};
+    j.wide = sc.wide.as<uint32_t>();
+    j.rec = sc.records.as<LeafRecord>();
+    j.slot_mat = sc.slot_mat.as<uint32_t>();
+    j.mats = sc.materials.as<rt_material>();
+    j.accum = accum;
+    j.nwide = sc.nwide;
+    j.width = width;
+    j.height = height;
+    j.spp = spp;
+    j.depth = p.maxRayTraceDepth;
+    j.random_state = p.randomState;
+    j.flags = flags;
+    j.tiles_x = (width + 7) / 8;
+    j.first_item = 0;
+    j.nitems = j.tiles_x * ((height + 7) / 8) * 64u;
+    camera_setup(p, width, height, j.cam);
+    return j;
+}
+
+void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,
+                   uint32_t flags, float *accum, unsigned long long *segments, const LaunchCfg &c,
+                   hipStream_t stream) {
+    RenderJob j = make_render_job(sc, p, width, height, spp, flags, accum);
     if (spp == 0) return;
-    if (sc.stack_bound > c.lds_stack + kSpillCap)
-        throw Error(RT_ERR_OVERFLOW, "traversal stack bound exceeds the kernel's capacity");
+    check_stack(sc, c);
     RenderArgs a{};
-    a.wide = sc.wide.as<uint32_t>();
-    a.rec = sc.records.as<LeafRecord>();
-    a.slot_mat = sc.slot_mat.as<uint32_t>();
-    a.mats = sc.materials.as<rt_material>();
-    a.nwide = sc.nwide;
-    a.width = width;
-    a.height = height;
-    a.spp = spp;
-    a.depth = p.maxRayTraceDepth;
-    a.random_state = p.randomState;
-    a.flags = flags;
-    a.refill = c.refill;
-    a.leaf_weight = c.leaf_weight;
-    a.tiles_x = (width + 7) / 8;
-    const uint64_t items = (uint64_t)a.tiles_x * ((height + 7) / 8) * 64u;
-    a.nitems = (uint32_t)items;
-    camera_setup(p, width, height, a.cam);
-    a.accum = accum;
+    a.job0 = j;
+    a.jobs = nullptr;
+    a.njobs = 1;
+    a.nitems = j.nitems;
     a.segments = segments;
-    a.work = ctx->counter_slot();
-    k_zero<<<1, 64, 0, stream>>>(a.work);
-    const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
-    void *k = pick_render(sc.width, c.lds_stack);
-    int per_cu = 0;
-    RT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)c.block, lds));
-    uint32_t grid = (uint32_t)std::max(1, per_cu) * (uint32_t)ctx->num_cus;
-    void *args[] = {&a};
-    RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
-    RT_HIP(hipGetLastError());
+    launch_render_kernel(ctx, a, sc.width, false, c, stream);
+}
+
+uint64_t launch_render_jobs(rt_context *ctx, std::vector<RenderJob> &jobs, uint32_t bvh_width,
+                            unsigned long long *segments, const LaunchCfg &c, hipStream_t stream) {
+    uint64_t total = 0;
+    for (RenderJob &j : jobs) {
+        total = (total + kRenderChunk - 1) / kRenderChunk * kRenderChunk;  // chunks never span jobs
+        j.first_item = (uint32_t)total;
+        total += j.spp ? j.nitems : 0u;  // a job with no samples gets no items
+    }
+    if (total == 0) return 0;
+    RT_REQUIRE(total < (1ull << 32) - kRenderChunk, "too many pixels in one launch");
+    int slot = 0;
+    void *table = ctx->tables.stage(jobs.data(), sizeof(RenderJob) * jobs.size(), stream, slot);
+    RenderArgs a{};
+    a.jobs = static_cast<const RenderJob *>(table);
+    a.njobs = (uint32_t)jobs.size();
+    a.nitems = (uint32_t)total;
+    a.segments = segments;
+    launch_render_kernel(ctx, a, bvh_width, true, c, stream);
+    ctx->tables.commit(slot, stream);
+    return total;
 }
 
 }  // namespace rtgpu

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -86,12 +87,87 @@ struct Scene {
     DevBuf binary;     // (2N-1) * rt_bvh_node when RT_BUILD_KEEP_BINARY
     bool has_tuned = false;
     LaunchCfg tuned_closest, tuned_any;
+    uint32_t users = 0;  // server requests holding this scene
     uint64_t device_bytes() const {
         return wide.bytes + records.bytes + slot_prim.bytes + slot_mat.bytes + materials.bytes + binary.bytes;
     }
 };
 
-struct rt_context_impl;
+// One image of a (possibly multi-image) render launch: where its scene lives, its
+// camera, and which slice of the launch's work items belongs to it.  Work items are
+// pixels in 8x8 tiles; each job's items start at a multiple of kRenderChunk so a
+// wave's chunk of items never spans two jobs.
+struct RenderJob {
+    const uint32_t *wide;
+    const LeafRecord *rec;
+    const uint32_t *slot_mat;
+    const rt_material *mats;
+    float *accum;
+    uint32_t nwide, width, height, spp;
+    uint32_t depth, random_state, flags, tiles_x;
+    uint32_t first_item, nitems, _pad0, _pad1;
+    float cam[12];  // pixel00, deltaU, deltaV, origin
+};
+constexpr uint32_t kRenderChunk = 64;  // work items (one 8x8 tile) per atomic on the render work counter
+
+// Small host->device tables (batch item lists, render job lists) that a launch reads.
+// Each slot has a pinned host buffer and a device buffer; a slot is reused only after
+// the launch that read it has finished (its event), so launches need no host sync.
+class TableRing {
+    static constexpr int kSlots = 32;
+    struct Slot {
+        void *host = nullptr;
+        DevBuf dev;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    Slot slots_[kSlots];
+    uint32_t next_ = 0;
+
+public:
+    TableRing() = default;
+    TableRing(const TableRing &) = delete;
+    TableRing &operator=(const TableRing &) = delete;
+    ~TableRing() {
+        for (Slot &s : slots_) {
+            if (s.done) {
+                (void)hipEventSynchronize(s.done);
+                (void)hipEventDestroy(s.done);
+            }
+            if (s.host) (void)hipHostFree(s.host);
+        }
+    }
+    // Enqueue a copy of `bytes` from `src` into a device table on `stream`.  Call
+    // commit(slot, stream) after the launch that reads the table.
+    void *stage(const void *src, size_t bytes, hipStream_t stream, int &slot) {
+        slot = (int)(next_++ % kSlots);
+        Slot &s = slots_[slot];
+        if (!s.done) RT_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        if (s.pending) RT_HIP(hipEventSynchronize(s.done));
+        s.pending = false;
+        if (s.cap < bytes) {
+            size_t cap = 4096;
+            while (cap < bytes) cap <<= 1;
+            if (s.host) RT_HIP(hipHostFree(s.host));
+            s.host = nullptr;
+            s.cap = 0;
+            RT_HIP(hipHostMalloc(&s.host, cap, hipHostMallocDefault));
+            s.dev.alloc(cap);
+            s.cap = cap;
+        }
+        std::memcpy(s.host, src, bytes);
+        RT_HIP(hipMemcpyAsync(s.dev.p, s.host, bytes, hipMemcpyHostToDevice, stream));
+        RT_HIP(hipEventRecord(s.done, stream));
+        s.pending = true;
+        return s.dev.p;
+    }
+    void commit(int slot, hipStream_t stream) {
+        Slot &s = slots_[slot];
+        RT_HIP(hipEventRecord(s.done, stream));
+        s.pending = true;
+    }
+};
 
 }  // namespace rtgpu
 
@@ -105,7 +181,10 @@ struct rt_context {
     rtgpu::DevBuf counters;
     uint32_t next_counter = 0;
     static constexpr uint32_t kCounterSlots = 4096;
-    std::mutex mu;  // guards scenes/counters for the server's worker thread
+    rtgpu::TableRing tables;
+    // Held by every entry point for its duration and by a server's worker thread while
+    // it enqueues a step, so a context may be shared by the caller and its servers.
+    std::recursive_mutex mu;
 
     rtgpu::Scene &scene(uint32_t id) {
         auto it = scenes.find(id);
@@ -127,16 +206,32 @@ void build_scene(rt_context *ctx, Scene &sc, const rt_triangle *tris, uint32_t n
 
 // trace.hip
 LaunchCfg resolve_cfg(const rt_context *ctx, const Scene &sc, const rt_launch_config *cfg, bool any);
+// throws RT_ERR_OVERFLOW when the tree's worst-case stack depth exceeds the kernel's capacity
+void check_stack(const Scene &sc, const LaunchCfg &cfg);
 void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t n, rt_hit *hits,
                   uint32_t *counts, uint32_t flags, const LaunchCfg &cfg, hipStream_t stream);
 void launch_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems, uint32_t flags,
                         const LaunchCfg &cfg, hipStream_t stream);
+struct TraceItem {
+    const Scene *sc;
+    const rt_ray *rays;
+    rt_hit *hits;
+    uint32_t n;
+};
+// One persistent launch per BVH width present over all items' rays; no host sync.
+void launch_trace_items(rt_context *ctx, const std::vector<TraceItem> &items, uint32_t flags, const LaunchCfg &cfg,
+                        hipStream_t stream);
 
 // render.hip
 void launch_accum_clear(float *accum, uint32_t npix, hipStream_t stream);
 void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height,
                    uint32_t spp, uint32_t flags, float *accum, unsigned long long *segments,
                    const LaunchCfg &cfg, hipStream_t stream);
+RenderJob make_render_job(const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,
+                          uint32_t flags, float *accum);
+// One launch over all jobs (all scenes must have BVH width `bvh_width`); returns the work items launched.
+uint64_t launch_render_jobs(rt_context *ctx, std::vector<RenderJob> &jobs, uint32_t bvh_width,
+                            unsigned long long *segments, const LaunchCfg &cfg, hipStream_t stream);
 void launch_resolve(const float *accum, uint32_t npix, uint32_t spp, float *out, hipStream_t stream);
 
 // host camera (raytraceBVH.comp:49-81); out: pixel00, deltaU, deltaV, origin

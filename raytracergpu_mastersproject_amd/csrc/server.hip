@@ -1,0 +1,582 @@
+// server.hip -- the serving loop: resident scenes, budgeted admission, continuous batching.
+//
+// Requests wait in a FIFO queue.  The head is admitted when a slot is free and the
+// request's buffers fit in the server's device arena (first-fit allocation in one
+// hipMalloc made at creation, so admission never calls the allocator).  Every admitted
+// request is advanced in scheduling steps: a step is one fused render launch over a
+// slice of samples of every admitted render (launch_render_jobs) plus one fused
+// ray-query launch over a slice of rays of every admitted query (launch_trace_items),
+// enqueued on the server's stream up to `pipeline` steps ahead of the device.  A
+// request that joins while others run takes its share from the next step on, and the
+// slices are sized so a step lasts about `step_us` of device time (rates measured
+// from earlier steps).  A request's last slice is followed by its resolve; when that
+// step's event completes the result is copied out and its arena range freed.
+//
+// A render's samples may be cut into any number of slices without changing its image
+// (each pixel's chain runs through its accumulator), and a ray's result does not
+// depend on the launch it is traced in, so results are the same alone or in a mix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "rt_internal.hpp"
+
+namespace rtgpu {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit allocator over [0, size) with 256-byte granularity.
+class Arena {
+    std::map<uint64_t, uint64_t> free_;  // offset -> length
+    uint64_t size_ = 0, used_ = 0, peak_ = 0;
+
+public:
+    static constexpr uint64_t kAlign = 256;
+    static uint64_t round(uint64_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+    void reset(uint64_t size) {
+        size_ = size / kAlign * kAlign;
+        free_.clear();
+        if (size_) free_[0] = size_;
+        used_ = peak_ = 0;
+    }
+    bool alloc(uint64_t n, uint64_t &off) {
+        n = round(std::max<uint64_t>(n, 1));
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            off = it->first;
+            const uint64_t rest = it->second - n;
+            free_.erase(it);
+            if (rest) free_[off + n] = rest;
+            used_ += n;
+            peak_ = std::max(peak_, used_);
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n) {
+        n = round(std::max<uint64_t>(n, 1));
+        used_ -= n;
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + n == next->first) {  // merge with the following range
+            n += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {  // merge with the preceding range
+                prev->second += n;
+                return;
+            }
+        }
+        free_[off] = n;
+    }
+    uint64_t size() const { return size_; }
+    uint64_t peak() const { return peak_; }
+};
+
+enum class State { Queued, Active, Finishing, Done };
+
+struct Request {
+    uint64_t id = 0;
+    rt_request req{};
+    State state = State::Queued;
+    Scene *scene = nullptr;
+    uint64_t bytes = 0, offset = 0;
+    uint32_t remaining = 0;  // samples (render) or rays (query) not yet given to a step
+    uint32_t done_units = 0;
+    rt_request_stats st{};
+};
+
+struct Step {
+    hipEvent_t start = nullptr, end = nullptr;
+    std::vector<std::shared_ptr<Request>> finishing;
+    std::vector<std::shared_ptr<Request>> members;
+    uint64_t samples = 0, rays = 0;
+};
+
+}  // namespace
+}  // namespace rtgpu
+
+struct rt_server {
+    rt_context *ctx = nullptr;
+    rt_server_config cfg{};
+    rtgpu::DevBuf arena_mem;
+    rtgpu::Arena arena;
+    rtgpu::DevBuf segments;  // u64 device counter of traced render segments
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> event_pool;
+
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<rtgpu::Request>> queue;
+    std::vector<std::shared_ptr<rtgpu::Request>> active;
+    std::deque<rtgpu::Step> inflight;
+    std::unordered_map<uint64_t, std::shared_ptr<rtgpu::Request>> all;
+    uint64_t next_id = 1;
+    bool stop = false;
+    int fatal = RT_OK;
+    std::string last_error;
+
+    // rates measured from retired steps (work units per microsecond of device time)
+    double sample_rate = 1000.0, ray_rate = 2000.0;
+    rt_server_stats stats{};
+    rtgpu::Clock::time_point t0;
+    std::thread worker;
+
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(rtgpu::Clock::now() - t0).count();
+    }
+};
+
+namespace rtgpu {
+namespace {
+
+uint64_t request_bytes(const rt_request &r) {
+    if (r.kind == RT_REQ_RENDER) return Arena::round(16ull * r.width * r.height);
+    return Arena::round(32ull * r.nrays) + Arena::round(16ull * r.nrays);
+}
+
+hipEvent_t take_event(rt_server *s) {
+    if (!s->event_pool.empty()) {
+        hipEvent_t e = s->event_pool.back();
+        s->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    RT_HIP(hipEventCreate(&e));
+    return e;
+}
+
+char *arena_ptr(rt_server *s, uint64_t off) { return static_cast<char *>(s->arena_mem.p) + off; }
+
+// Admits queue heads while a slot and arena space are free.  Called with s->m held.
+void admit(rt_server *s) {
+    const uint32_t slots = (s->cfg.flags & RT_SERVER_SERIAL) ? std::max(1u, s->cfg.pipeline) : s->cfg.max_active;
+    while (!s->queue.empty() && s->active.size() < slots) {
+        std::shared_ptr<Request> r = s->queue.front();
+        uint64_t off = 0;
+        if (!s->arena.alloc(r->bytes, off)) break;  // arrival order: the head waits for memory
+        s->queue.pop_front();
+        r->offset = off;
+        r->state = State::Active;
+        r->st.admit_ms = s->now_ms();
+        {
+            std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+            if (r->req.kind == RT_REQ_RENDER) {
+                launch_accum_clear(reinterpret_cast<float *>(arena_ptr(s, off)), r->req.width * r->req.height,
+                                   s->stream);
+            } else if (r->req.nrays) {
+                RT_HIP(hipMemcpyAsync(arena_ptr(s, off), r->req.rays, 32ull * r->req.nrays, hipMemcpyHostToDevice,
+                                      s->stream));
+            }
+        }
+        s->active.push_back(r);
+    }
+}
+
+// Builds and enqueues one step.  Called with s->m held; returns false when nothing was scheduled.
+bool enqueue_step(rt_server *s) {
+    std::vector<std::shared_ptr<Request>> renders, queries;
+    for (auto &r : s->active)
+        if (r->state == State::Active && r->remaining > 0)
+            (r->req.kind == RT_REQ_RENDER ? renders : queries).push_back(r);
+    if (renders.empty() && queries.empty()) return false;
+    const bool serial = (s->cfg.flags & RT_SERVER_SERIAL) != 0;
+    if (serial) {  // the oldest admitted request, whole
+        std::shared_ptr<Request> first;
+        for (auto &r : s->active)
+            if (r->state == State::Active && r->remaining > 0) { first = r; break; }
+        renders.clear();
+        queries.clear();
+        (first->req.kind == RT_REQ_RENDER ? renders : queries).push_back(first);
+    }
+    Step step;
+    step.start = take_event(s);
+    step.end = take_event(s);
+    std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+    RT_HIP(hipEventRecord(step.start, s->stream));
+    // samples per render this step: an equal share of the step's budget, at least one pass
+    const double budget_samples = s->sample_rate * s->cfg.step_us;
+    const double share = budget_samples / (double)std::max<size_t>(1, renders.size());
+    std::map<uint32_t, std::vector<RenderJob>> by_width;
+    LaunchCfg rc;
+    bool rc_set = false;
+    for (auto &r : renders) {
+        const uint64_t npix = (uint64_t)r->req.width * r->req.height;
+        uint32_t q = serial ? r->remaining
+                            : (uint32_t)std::min<double>(r->remaining, std::max(1.0, std::floor(share / (double)npix)));
+        q = std::max(1u, std::min(q, r->remaining));
+        float *accum = reinterpret_cast<float *>(arena_ptr(s, r->offset));
+        const uint32_t kflags = r->req.render_flags & ~RT_REQ_OUT_ACCUM;
+        if (!rc_set) {
+            rc = resolve_cfg(s->ctx, *r->scene, &s->cfg.render_cfg, false);
+            if (!s->cfg.render_cfg.persistent) rc.persistent = 1;
+            rc_set = true;
+        }
+        check_stack(*r->scene, rc);
+        if (serial) {
+            launch_render(s->ctx, *r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum,
+                          s->segments.as<unsigned long long>(), rc, s->stream);
+        } else {
+            by_width[r->scene->width].push_back(
+                make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum));
+        }
+        r->remaining -= q;
+        r->done_units += q;
+        r->st.steps++;
+        step.samples += npix * q;
+        step.members.push_back(r);
+    }
+    for (auto &kv : by_width)
+        launch_render_jobs(s->ctx, kv.second, kv.first, s->segments.as<unsigned long long>(), rc, s->stream);
+    if (!queries.empty()) {
+        const double rshare = s->ray_rate * s->cfg.step_us / (double)queries.size();
+        std::vector<TraceItem> closest, any;
+        for (auto &r : queries) {
+            uint32_t q = serial ? r->remaining
+                                : (uint32_t)std::min<double>(r->remaining, std::max(65536.0, std::floor(rshare)));
+            q = std::max(1u, std::min(q, r->remaining));
+            const uint32_t first = r->done_units;
+            const rt_ray *rays = reinterpret_cast<const rt_ray *>(arena_ptr(s, r->offset)) + first;
+            rt_hit *hits = reinterpret_cast<rt_hit *>(arena_ptr(s, r->offset + Arena::round(32ull * r->req.nrays))) + first;
+            ((r->req.trace_flags & RT_TRACE_ANY) ? any : closest).push_back(TraceItem{r->scene, rays, hits, q});
+            r->remaining -= q;
+            r->done_units += q;
+            r->st.steps++;
+            step.rays += q;
+            step.members.push_back(r);
+        }
+        for (int pass = 0; pass < 2; ++pass) {
+            auto &items = pass ? any : closest;
+            if (items.empty()) continue;
+            const uint32_t flags = pass ? RT_TRACE_ANY : RT_TRACE_CLOSEST;
+            LaunchCfg tc = resolve_cfg(s->ctx, *items[0].sc, &s->cfg.trace_cfg, pass != 0);
+            if (!s->cfg.trace_cfg.persistent) tc.persistent = 1;
+            if (serial) launch_trace(s->ctx, *items[0].sc, items[0].rays, items[0].n, items[0].hits, nullptr, flags, tc,
+                                     s->stream);
+            else launch_trace_items(s->ctx, items, flags, tc, s->stream);
+        }
+    }
+    for (auto &r : step.members) {
+        if (r->remaining) continue;
+        r->state = State::Finishing;
+        if (r->req.kind == RT_REQ_RENDER && !(r->req.render_flags & RT_REQ_OUT_ACCUM)) {
+            float *accum = reinterpret_cast<float *>(arena_ptr(s, r->offset));
+            launch_resolve(accum, r->req.width * r->req.height, r->req.spp, accum, s->stream);  // in place
+        }
+        step.finishing.push_back(r);
+    }
+    RT_HIP(hipEventRecord(step.end, s->stream));
+    s->stats.steps++;
+    s->inflight.push_back(std::move(step));
+    return true;
+}
+
+// Completes the oldest step (its end event has fired).  Called with s->m held.
+void retire_front(rt_server *s, double done_ms) {
+    Step step = std::move(s->inflight.front());
+    s->inflight.pop_front();
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, step.start, step.end) == hipSuccess && ms > 0.0f) {
+        s->stats.busy_ms += ms;
+        const double us = 1000.0 * ms;
+        if (step.samples && !step.rays) s->sample_rate = 0.5 * s->sample_rate + 0.5 * (double)step.samples / us;
+        if (step.rays && !step.samples) s->ray_rate = 0.5 * s->ray_rate + 0.5 * (double)step.rays / us;
+    }
+    s->stats.samples += step.samples;
+    s->stats.rays += step.rays;
+    for (auto &r : step.finishing) {
+        if (r->req.out) {
+            const uint64_t off = r->req.kind == RT_REQ_RENDER ? r->offset : r->offset + Arena::round(32ull * r->req.nrays);
+            const uint64_t n = r->req.kind == RT_REQ_RENDER ? 16ull * r->req.width * r->req.height : 16ull * r->req.nrays;
+            if (n) RT_HIP(hipMemcpy(r->req.out, arena_ptr(s, off), n, hipMemcpyDeviceToHost));
+        }
+        {
+            std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+            r->scene->users--;
+        }
+        s->arena.release(r->offset, r->bytes);
+        r->state = State::Done;
+        r->st.done_ms = done_ms;
+        r->st.status = RT_OK;
+        s->stats.completed++;
+        s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+    }
+    s->event_pool.push_back(step.start);
+    s->event_pool.push_back(step.end);
+    s->cv_done.notify_all();
+}
+
+void fail_all(rt_server *s, int status, const std::string &msg) {
+    s->fatal = status;
+    s->last_error = msg;
+    auto fail = [&](std::shared_ptr<Request> &r) {
+        if (r->state == State::Done) return;
+        r->state = State::Done;
+        r->st.status = status;
+        r->st.done_ms = s->now_ms();
+        s->stats.failed++;
+    };
+    for (auto &r : s->queue) fail(r);
+    for (auto &r : s->active) fail(r);
+    {
+        std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+        for (auto &r : s->queue) r->scene->users--;
+        for (auto &r : s->active) r->scene->users--;
+    }
+    s->queue.clear();
+    s->active.clear();
+    s->inflight.clear();
+    s->cv_done.notify_all();
+}
+
+void worker_main(rt_server *s) {
+    std::unique_lock<std::mutex> lk(s->m);
+    try {
+        RT_HIP(hipSetDevice(s->ctx->device));
+        for (;;) {
+            // retire steps that have finished
+            while (!s->inflight.empty() && hipEventQuery(s->inflight.front().end) == hipSuccess)
+                retire_front(s, s->now_ms());
+            admit(s);
+            if (s->inflight.size() < s->cfg.pipeline && enqueue_step(s)) continue;
+            if (!s->inflight.empty()) {
+                hipEvent_t e = s->inflight.front().end;
+                lk.unlock();
+                const hipError_t err = hipEventSynchronize(e);
+                const double t = s->now_ms();
+                lk.lock();
+                RT_HIP(err);
+                retire_front(s, t);
+                continue;
+            }
+            if (s->stop && s->queue.empty() && s->active.empty()) break;
+            if (!s->queue.empty() && s->active.empty())  // cannot happen: an empty arena fits any admitted size
+                throw Error(RT_ERR_BUDGET, "queued request does not fit the arena");
+            s->cv_work.wait(lk);
+        }
+    } catch (const Error &e) {
+        fail_all(s, e.status, e.what());
+    } catch (const std::exception &e) {
+        fail_all(s, RT_ERR_HIP, e.what());
+    }
+}
+
+}  // namespace
+}  // namespace rtgpu
+
+using namespace rtgpu;
+
+extern "C" {
+
+int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **out) {
+    if (!ctx || !out) return RT_ERR_INVALID;
+    *out = nullptr;
+    std::unique_ptr<rt_server> s;
+    try {
+        s = std::make_unique<rt_server>();
+        s->ctx = ctx;
+        if (cfg) s->cfg = *cfg;
+        if (!s->cfg.arena_bytes) s->cfg.arena_bytes = 4ull << 30;
+        if (!s->cfg.max_active) s->cfg.max_active = 64;
+        if (!s->cfg.step_us) s->cfg.step_us = 4000;
+        if (!s->cfg.pipeline) s->cfg.pipeline = 2;
+        s->cfg.pipeline = std::min(s->cfg.pipeline, 8u);
+        RT_REQUIRE((s->cfg.flags & ~RT_SERVER_SERIAL) == 0, "unknown server flags");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        RT_HIP(hipSetDevice(ctx->device));
+        s->arena_mem.alloc(s->cfg.arena_bytes);
+        s->arena.reset(s->cfg.arena_bytes);
+        s->segments.alloc(8);
+        RT_HIP(hipMemset(s->segments.p, 0, 8));
+        // non-blocking: the server's work does not serialise with the legacy default stream
+        RT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->stats.arena_bytes = s->arena.size();
+        s->t0 = Clock::now();
+        s->worker = std::thread(worker_main, s.get());
+    } catch (const Error &e) {
+        ctx->last_error = e.what();
+        if (s && s->stream) (void)hipStreamDestroy(s->stream);
+        return e.status;
+    } catch (const std::exception &e) {
+        ctx->last_error = e.what();
+        if (s && s->stream) (void)hipStreamDestroy(s->stream);
+        return RT_ERR_INVALID;
+    }
+    *out = s.release();
+    return RT_OK;
+}
+
+int rt_server_destroy(rt_server *s) {
+    if (!s) return RT_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    const int st = s->fatal;
+    (void)hipStreamSynchronize(s->stream);
+    for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(s->stream);
+    delete s;
+    return st;
+}
+
+int rt_server_submit(rt_server *s, const rt_request *req, uint64_t *id) {
+    if (!s || !req) return RT_ERR_INVALID;
+    auto r = std::make_shared<Request>();
+    r->req = *req;
+    try {
+        RT_REQUIRE(req->kind == RT_REQ_RENDER || req->kind == RT_REQ_TRACE, "unknown request kind");
+        {
+            std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+            r->scene = &s->ctx->scene(req->scene_id);
+        }
+        if (req->kind == RT_REQ_RENDER) {
+            RT_REQUIRE((req->render_flags & ~(RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE |
+                                              RT_REQ_OUT_ACCUM)) == 0,
+                       "unknown render flags");
+            RT_REQUIRE(req->width > 0 && req->height > 0 && (uint64_t)req->width * req->height < (1ull << 31),
+                       "bad image size");
+            RT_REQUIRE(req->spp > 0, "a render request needs at least one sample per pixel");
+            r->remaining = req->spp;
+        } else {
+            RT_REQUIRE((req->trace_flags & ~RT_TRACE_ANY) == 0, "unknown trace flags");
+            RT_REQUIRE(req->nrays == 0 || req->rays, "rays is NULL");
+            r->remaining = req->nrays;
+        }
+        {
+            // a tree too deep for the configured stack is refused here rather than failing a step
+            std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+            const bool render = req->kind == RT_REQ_RENDER;
+            check_stack(*r->scene, resolve_cfg(s->ctx, *r->scene, render ? &s->cfg.render_cfg : &s->cfg.trace_cfg,
+                                               !render && (req->trace_flags & RT_TRACE_ANY)));
+        }
+        r->bytes = request_bytes(*req);
+    } catch (const Error &e) {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->last_error = e.what();
+        return e.status;
+    }
+    std::lock_guard<std::mutex> lk(s->m);
+    if (s->fatal != RT_OK) return s->fatal;
+    if (r->bytes > s->arena.size()) {
+        s->last_error = "request needs " + std::to_string(r->bytes) + " bytes; the arena holds " +
+                        std::to_string(s->arena.size());
+        return RT_ERR_BUDGET;
+    }
+    if (r->remaining) {  // the scene stays resident until the request completes
+        std::lock_guard<std::recursive_mutex> clk(s->ctx->mu);
+        auto it = s->ctx->scenes.find(req->scene_id);  // looked up again: it may have been released meanwhile
+        if (it == s->ctx->scenes.end()) {
+            s->last_error = "unknown scene id " + std::to_string(req->scene_id);
+            return RT_ERR_NOT_FOUND;
+        }
+        r->scene = it->second.get();
+        r->scene->users++;
+    }
+    r->id = s->next_id++;
+    r->st.id = r->id;
+    r->st.device_bytes = r->bytes;
+    r->st.submit_ms = s->now_ms();
+    s->stats.submitted++;
+    s->all[r->id] = r;
+    if (r->remaining == 0) {  // an empty ray query completes at once
+        r->state = State::Done;
+        r->st.admit_ms = r->st.done_ms = r->st.submit_ms;
+        r->st.status = RT_OK;
+        s->stats.completed++;
+        if (id) *id = r->id;
+        s->cv_done.notify_all();
+        return RT_OK;
+    }
+    s->queue.push_back(r);
+    if (id) *id = r->id;
+    s->cv_work.notify_all();
+    return RT_OK;
+}
+
+int rt_server_wait(rt_server *s, uint64_t id, int timeout_ms, rt_request_stats *stats) {
+    if (!s) return RT_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->m);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) {
+        s->last_error = "unknown request id " + std::to_string(id);
+        return RT_ERR_NOT_FOUND;
+    }
+    std::shared_ptr<Request> r = it->second;
+    auto done = [&] { return r->state == State::Done; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, done);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done)) return RT_ERR_BUSY;
+    if (stats) *stats = r->st;
+    return r->st.status;
+}
+
+int rt_server_drain(rt_server *s) {
+    if (!s) return RT_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->m);
+    s->cv_done.wait(lk, [&] { return (s->queue.empty() && s->active.empty()) || s->fatal != RT_OK; });
+    return s->fatal;
+}
+
+int rt_server_stats_get(rt_server *s, rt_server_stats *out) {
+    if (!s || !out) return RT_ERR_INVALID;
+    unsigned long long segs = 0;
+    if (hipMemcpy(&segs, s->segments.p, 8, hipMemcpyDeviceToHost) != hipSuccess) return RT_ERR_HIP;
+    std::lock_guard<std::mutex> lk(s->m);
+    *out = s->stats;
+    out->segments = segs;
+    out->arena_peak = s->arena.peak();
+    out->wall_ms = s->now_ms();
+    out->active = (uint32_t)s->active.size();
+    out->queued = (uint32_t)s->queue.size();
+    return RT_OK;
+}
+
+int rt_server_replay(rt_server *s, const rt_request *reqs, const double *arrival_ms, uint32_t n,
+                     rt_request_stats *out) {
+    if (!s || (n && (!reqs || !arrival_ms))) return RT_ERR_INVALID;
+    std::vector<uint64_t> ids(n, 0);
+    std::vector<int> status(n, RT_OK);
+    double base;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        base = s->now_ms();
+    }
+    const Clock::time_point start = Clock::now();
+    for (uint32_t i = 0; i < n; ++i) {
+        std::this_thread::sleep_until(start + std::chrono::duration_cast<Clock::duration>(
+                                                  std::chrono::duration<double, std::milli>(arrival_ms[i])));
+        status[i] = rt_server_submit(s, &reqs[i], &ids[i]);
+    }
+    int result = RT_OK;
+    for (uint32_t i = 0; i < n; ++i) {
+        rt_request_stats st{};
+        if (status[i] == RT_OK) status[i] = rt_server_wait(s, ids[i], -1, &st);
+        if (status[i] != RT_OK && result == RT_OK) result = status[i];
+        if (out) {
+            st.id = ids[i];
+            st.status = status[i];
+            // open loop: latency runs from the scheduled arrival, not from when submit returned
+            st.submit_ms = arrival_ms[i];
+            st.admit_ms = st.admit_ms ? st.admit_ms - base : 0.0;
+            st.done_ms = st.done_ms ? st.done_ms - base : 0.0;
+            out[i] = st;
+        }
+    }
+    return result;
+}
+
+const char *rt_server_last_error(const rt_server *s) { return s ? s->last_error.c_str() : "null server"; }
+
+}  // extern "C"

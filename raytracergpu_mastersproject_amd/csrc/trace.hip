@@ -248,13 +248,13 @@ uint32_t persistent_grid(rt_context *ctx, const void *kernel, uint32_t block, si
     return (uint32_t)std::max(1, per_cu) * (uint32_t)ctx->num_cus;
 }
 
+}  // namespace
+
 void check_stack(const Scene &sc, const LaunchCfg &c) {
     if (sc.stack_bound > c.lds_stack + kSpillCap)
         throw Error(RT_ERR_OVERFLOW, "traversal stack bound " + std::to_string(sc.stack_bound) +
                                          " exceeds the kernel's capacity " + std::to_string(c.lds_stack + kSpillCap));
 }
-
-}  // namespace
 
 LaunchCfg resolve_cfg(const rt_context *ctx, const Scene &sc, const rt_launch_config *cfg, bool any) {
     (void)ctx;
@@ -296,36 +296,47 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
     RT_HIP(hipGetLastError());
 }
 
+void launch_trace_items(rt_context *ctx, const std::vector<TraceItem> &items, uint32_t flags, const LaunchCfg &c,
+                        hipStream_t stream) {
+    // one launch per BVH width present, each over that width's items in order
+    for (uint32_t width : {2u, 4u, 8u}) {
+        std::vector<BatchItemDev> hv;
+        uint64_t total = 0;
+        for (const TraceItem &it : items) {
+            if (it.sc->width != width) continue;
+            RT_REQUIRE(it.n == 0 || (it.rays && it.hits), "batch item with NULL rays/hits");
+            check_stack(*it.sc, c);
+            hv.push_back(BatchItemDev{view_of(*it.sc), it.rays, it.hits, total});
+            total += it.n;
+        }
+        if (total == 0) continue;
+        int slot = 0;
+        void *table = ctx->tables.stage(hv.data(), sizeof(BatchItemDev) * hv.size(), stream, slot);
+        unsigned long long *work = reinterpret_cast<unsigned long long *>(ctx->counter_slot());
+        k_zero_counter<<<1, 64, 0, stream>>>(reinterpret_cast<unsigned int *>(work));
+        BatchArgs a{static_cast<const BatchItemDev *>(table), (uint32_t)hv.size(), c.refill, total, work,
+                    c.leaf_weight};
+        const bool any = (flags & RT_TRACE_ANY) != 0;
+        const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
+        void *k = pick_kernel(width, c.lds_stack, 2, any);
+        const uint64_t chunks = (total + kChunk - 1) / kChunk, wpb = c.block / 64u;
+        const uint32_t grid = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(persistent_grid(ctx, k, c.block, lds), (chunks + wpb - 1) / wpb));
+        void *args[] = {&a};
+        RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
+        RT_HIP(hipGetLastError());
+        ctx->tables.commit(slot, stream);
+    }
+}
+
 void launch_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems, uint32_t flags,
                         const LaunchCfg &c, hipStream_t stream) {
     if (nitems == 0) return;
     RT_REQUIRE(items, "items is NULL");
-    std::vector<BatchItemDev> hv(nitems);
-    uint64_t total = 0;
-    uint32_t width = 0;
-    for (uint32_t i = 0; i < nitems; ++i) {
-        const Scene &sc = ctx->scene(items[i].scene_id);
-        if (i == 0) width = sc.width;
-        RT_REQUIRE(sc.width == width, "rt_trace_batch: all scenes must share one BVH width");
-        RT_REQUIRE(items[i].nrays == 0 || (items[i].rays && items[i].hits), "rt_trace_batch: NULL rays/hits");
-        check_stack(sc, c);
-        hv[i] = BatchItemDev{view_of(sc), items[i].rays, items[i].hits, total};
-        total += items[i].nrays;
-    }
-    if (total == 0) return;
-    DevBuf ditems(sizeof(BatchItemDev) * nitems);
-    RT_HIP(hipMemcpyAsync(ditems.p, hv.data(), sizeof(BatchItemDev) * nitems, hipMemcpyHostToDevice, stream));
-    unsigned long long *work = reinterpret_cast<unsigned long long *>(ctx->counter_slot());
-    k_zero_counter<<<1, 64, 0, stream>>>(reinterpret_cast<unsigned int *>(work));
-    BatchArgs a{ditems.as<BatchItemDev>(), nitems, c.refill, total, work, c.leaf_weight};
-    const bool any = (flags & RT_TRACE_ANY) != 0;
-    const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
-    void *k = pick_kernel(width, c.lds_stack, 2, any);
-    uint32_t grid = persistent_grid(ctx, k, c.block, lds);
-    void *args[] = {&a};
-    RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
-    RT_HIP(hipGetLastError());
-    RT_HIP(hipStreamSynchronize(stream));  // the item table is freed on return
+    std::vector<TraceItem> v(nitems);
+    for (uint32_t i = 0; i < nitems; ++i)
+        v[i] = TraceItem{&ctx->scene(items[i].scene_id), items[i].rays, items[i].hits, items[i].nrays};
+    launch_trace_items(ctx, v, flags, c, stream);
 }
 
 }  // namespace rtgpu

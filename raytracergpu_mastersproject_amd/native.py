@@ -58,6 +58,41 @@ class TuneResult(C.Structure):
                 ("_pad", C.c_uint32)]
 
 
+class RenderJob(C.Structure):
+    _fields_ = [("scene_id", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("flags", C.c_uint32), ("_reserved", C.c_uint32), ("accum", C.c_void_p), ("params", C.c_ubyte * 80)]
+
+
+REQ_RENDER = 0
+REQ_TRACE = 1
+REQ_OUT_ACCUM = 0x100
+SERVER_SERIAL = 1
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("arena_bytes", C.c_uint64), ("max_active", C.c_uint32), ("step_us", C.c_uint32),
+                ("pipeline", C.c_uint32), ("flags", C.c_uint32), ("render_cfg", LaunchConfig),
+                ("trace_cfg", LaunchConfig)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("scene_id", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("spp", C.c_uint32), ("render_flags", C.c_uint32), ("params", C.c_ubyte * 80),
+                ("nrays", C.c_uint32), ("trace_flags", C.c_uint32), ("rays", C.c_void_p), ("out", C.c_void_p)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double),
+                ("device_bytes", C.c_uint64), ("steps", C.c_uint32), ("status", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64), ("steps", C.c_uint64),
+                ("samples", C.c_uint64), ("segments", C.c_uint64), ("rays", C.c_uint64),
+                ("arena_bytes", C.c_uint64), ("arena_peak", C.c_uint64), ("busy_ms", C.c_double),
+                ("wall_ms", C.c_double), ("active", C.c_uint32), ("queued", C.c_uint32)]
+
+
 _vp, _u32, _i32, _sz = C.c_void_p, C.c_uint32, C.c_int, C.c_size_t
 _SIGS = {
     "rt_create": (_i32, [C.POINTER(_vp), _i32]),
@@ -76,6 +111,15 @@ _SIGS = {
     "rt_resolve": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp]),
     "rt_trace_batch": (_i32, [_vp, C.POINTER(BatchItem), _u32, _u32, C.POINTER(LaunchConfig), _vp]),
     "rt_autotune": (_i32, [_vp, _u32, _vp, _u32, _vp, _u32, C.POINTER(TuneResult)]),
+    "rt_render_batch": (_i32, [_vp, C.POINTER(RenderJob), _u32, _vp, C.POINTER(LaunchConfig), _vp]),
+    "rt_server_create": (_i32, [_vp, C.POINTER(ServerConfig), C.POINTER(_vp)]),
+    "rt_server_destroy": (_i32, [_vp]),
+    "rt_server_submit": (_i32, [_vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "rt_server_wait": (_i32, [_vp, C.c_uint64, _i32, C.POINTER(RequestStats)]),
+    "rt_server_drain": (_i32, [_vp]),
+    "rt_server_stats_get": (_i32, [_vp, C.POINTER(ServerStats)]),
+    "rt_server_replay": (_i32, [_vp, C.POINTER(Request), C.POINTER(C.c_double), _u32, C.POINTER(RequestStats)]),
+    "rt_server_last_error": (C.c_char_p, [_vp]),
     "rt_util_malloc": (_i32, [C.POINTER(_vp), _sz]),
     "rt_util_free": (_i32, [_vp]),
     "rt_util_memcpy_htod": (_i32, [_vp, _vp, _sz]),
@@ -330,3 +374,108 @@ class Context:
 
     def resolve(self, accum: DeviceArray, npix: int, spp: int, out: DeviceArray, stream=None):
         self._ck(lib.rt_resolve(self.h, _ptr(accum), npix, spp, _ptr(out), stream.handle if stream else None))
+
+    def render_batch(self, jobs, segments: DeviceArray | None = None, cfg=None, stream=None):
+        """jobs: (scene_id, params, width, height, spp, accum[, flags]) tuples, one launch per BVH width."""
+        arr = (RenderJob * len(jobs))()
+        for k, j in enumerate(jobs):
+            sid, params, w, h, spp, accum = j[:6]
+            flags = j[6] if len(j) > 6 else 0
+            arr[k].scene_id, arr[k].width, arr[k].height, arr[k].spp, arr[k].flags = sid, w, h, spp, flags
+            arr[k].accum = _ptr(accum)
+            C.memmove(arr[k].params, np.ascontiguousarray(params, dtype=L.PARAMS).ctypes.data, 80)
+        self._ck(lib.rt_render_batch(self.h, arr, len(jobs), _ptr(segments), C.byref(cfg) if cfg else None,
+                                     stream.handle if stream else None))
+
+
+def render_request(scene_id, params, width, height, spp, flags=0, out: np.ndarray | None = None) -> Request:
+    r = Request()
+    r.kind, r.scene_id, r.width, r.height, r.spp, r.render_flags = REQ_RENDER, scene_id, width, height, spp, flags
+    C.memmove(r.params, np.ascontiguousarray(params, dtype=L.PARAMS).ctypes.data, 80)
+    if out is not None:
+        assert out.flags.c_contiguous and out.nbytes >= 16 * width * height
+        r.out = out.ctypes.data
+    return r
+
+
+def trace_request(scene_id, rays: np.ndarray, any_hit=False, out: np.ndarray | None = None) -> Request:
+    rays = np.ascontiguousarray(rays, dtype=L.RAY)
+    r = Request()
+    r.kind, r.scene_id, r.nrays = REQ_TRACE, scene_id, len(rays)
+    r.trace_flags = TRACE_ANY if any_hit else TRACE_CLOSEST
+    r.rays = rays.ctypes.data if len(rays) else None
+    r._keep = rays  # host rays must live until admission
+    if out is not None:
+        assert out.flags.c_contiguous and out.nbytes >= 16 * len(rays)
+        r.out = out.ctypes.data
+    return r
+
+
+class Server:
+    """A serving loop over a context's resident scenes (rt_server_*)."""
+
+    def __init__(self, ctx: Context, arena_bytes=1 << 30, max_active=64, step_us=4000, pipeline=2, serial=False,
+                 render_cfg: LaunchConfig | None = None, trace_cfg: LaunchConfig | None = None):
+        cfg = ServerConfig(arena_bytes, max_active, step_us, pipeline, SERVER_SERIAL if serial else 0)
+        if render_cfg:
+            cfg.render_cfg = render_cfg
+        if trace_cfg:
+            cfg.trace_cfg = trace_cfg
+        h = C.c_void_p()
+        self.ctx = ctx
+        _check(lib.rt_server_create(ctx.h, C.byref(cfg), C.byref(h)), ctx.h)
+        self.h = h.value
+        self._pending = {}
+
+    def _ck(self, status):
+        if status != RT_OK:
+            raise RTError(status, lib.rt_server_last_error(self.h).decode())
+
+    def submit(self, req: Request) -> int:
+        rid = C.c_uint64()
+        self._ck(lib.rt_server_submit(self.h, C.byref(req), C.byref(rid)))
+        self._pending[rid.value] = req  # keeps host buffers alive until completion
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> RequestStats:
+        st = RequestStats()
+        status = lib.rt_server_wait(self.h, rid, timeout_ms, C.byref(st))
+        if status != RT_ERR_BUSY_CODE:
+            self._pending.pop(rid, None)
+        self._ck(status)
+        return st
+
+    def drain(self):
+        self._ck(lib.rt_server_drain(self.h))
+        self._pending.clear()
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        self._ck(lib.rt_server_stats_get(self.h, C.byref(st)))
+        return st
+
+    def replay(self, reqs, arrival_ms) -> list:
+        n = len(reqs)
+        arr = (Request * n)(*reqs)
+        arrivals = np.ascontiguousarray(arrival_ms, dtype=np.float64)
+        out = (RequestStats * n)()
+        status = lib.rt_server_replay(self.h, arr, arrivals.ctypes.data_as(C.POINTER(C.c_double)), n, out)
+        self._ck(status)
+        return list(out)
+
+    def close(self):
+        if getattr(self, "h", None):
+            h, self.h = self.h, None
+            status = lib.rt_server_destroy(h)
+            self._pending.clear()
+            if status != RT_OK:
+                raise RTError(status, "server failed")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+RT_ERR_BUSY_CODE = 7

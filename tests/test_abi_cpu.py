@@ -115,3 +115,41 @@ def test_workload_sizes_of_the_headline_configs():
     from raytracergpu_mastersproject_amd import workloads as W
     assert 20 * 224 ** 2 == 1_003_520
     assert 64 * 65536 == 4_194_304
+
+
+def test_ctypes_structs_match_the_c_header():
+    """Every struct the Python binding passes across the ABI has the header's size and field offsets."""
+    import ctypes as C
+    from raytracergpu_mastersproject_amd import native as N
+    structs = {"rt_build_options": N.BuildOptions, "rt_launch_config": N.LaunchConfig, "rt_scene_info": N.SceneInfo,
+               "rt_batch_item": N.BatchItem, "rt_tune_result": N.TuneResult, "rt_render_job": N.RenderJob,
+               "rt_server_config": N.ServerConfig, "rt_request": N.Request, "rt_request_stats": N.RequestStats,
+               "rt_server_stats": N.ServerStats}
+    lines = []
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    prog = "#include <stdio.h>\n#include <stddef.h>\n#include \"rtgpu.h\"\nint main(void) {\n" + "\n".join(lines) + \
+        "\nreturn 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = Path(d) / "s.c", Path(d) / "s"
+        src.write_text(prog)
+        subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+        got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True,
+                                                                    text=True).stdout.split("\n") if line)
+    for cname, py in structs.items():
+        assert C.sizeof(py) == int(got[cname]), cname
+        for f, _ in py._fields_:
+            assert getattr(py, f).offset == int(got[f"{cname}.{f}"]), f"{cname}.{f}"
+
+
+def test_server_entry_points_reject_null_handles():
+    from raytracergpu_mastersproject_amd import native as N
+    assert N.lib.rt_server_create(None, None, None) == 1
+    assert N.lib.rt_server_submit(None, None, None) == 1
+    assert N.lib.rt_server_wait(None, 1, 0, None) == 1
+    assert N.lib.rt_server_drain(None) == 1
+    assert N.lib.rt_server_destroy(None) == 1
+    assert N.lib.rt_server_last_error(None) == b"null server"
+    assert N.lib.rt_render_batch(None, None, 0, None, None, None) == 1

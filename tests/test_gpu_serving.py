@@ -1,0 +1,220 @@
+"""Batched rendering (many images in one launch) and the serving loop on the GPU.
+
+Every image rendered inside a batch or by the server must be bit-identical to the same image
+rendered alone with rt_render (whose images are pinned to the oracle in test_gpu_render.py), and every
+ray query served in a mix must return exactly the hits of rt_trace alone."""
+import numpy as np
+import pytest
+
+from raytracergpu_mastersproject_amd import layouts as L
+from raytracergpu_mastersproject_amd import workloads as W
+
+from helpers import gpu_trace, small_cluster_scene, upload_both
+
+pytestmark = pytest.mark.gpu
+
+
+def render_alone(native, ctx, sid, params, w, h, spp, flags=0):
+    acc = native.DeviceArray(16 * w * h)
+    ctx.accum_clear(acc, w * h)
+    ctx.render(sid, params, w, h, spp, acc, flags=flags)
+    return acc.to_numpy(np.float32, 4 * w * h).reshape(-1, 4)
+
+
+def with_state(params, random_state, depth=None):
+    p = params.copy()
+    p["randomState"] = random_state
+    if depth is not None:
+        p["maxRayTraceDepth"] = depth
+    return p
+
+
+@pytest.fixture(scope="module")
+def scenes(ctx, native):
+    soup = W.soup_scene()
+    ico = W.icosphere_scene(freq=12)
+    ids = {
+        "soup4": ctx.upload(soup["tris"], None, soup["materials"], width=4),
+        "soup8": ctx.upload(soup["tris"], None, soup["materials"], width=8, leaf_size=2),
+        "ico": ctx.upload(ico["tris"], None, ico["materials"], width=4),
+        "ico2": ctx.upload(ico["tris"], None, ico["materials"], width=2),
+    }
+    params = {"soup4": soup["params"], "soup8": soup["params"], "ico": ico["params"], "ico2": ico["params"]}
+    yield ids, params, soup, ico
+    for sid in ids.values():
+        ctx.release(sid)
+
+
+JOBS = [("soup4", 64, 48, 3, 0, 0), ("ico", 96, 64, 2, 0, 7), ("soup8", 40, 40, 1, 0, 3),
+        ("soup4", 33, 17, 5, 2, 11), ("ico2", 16, 16, 4, 1, 5), ("ico", 8, 8, 9, 0, 1)]
+
+
+def test_render_batch_images_equal_single_renders(ctx, native, O, scenes):
+    ids, params, soup, _ = scenes
+    accs, jobs, alone = [], [], []
+    for name, w, h, spp, flags, rs in JOBS:
+        p = with_state(params[name], rs)
+        acc = native.DeviceArray(16 * w * h)
+        ctx.accum_clear(acc, w * h)
+        accs.append(acc)
+        jobs.append((ids[name], p, w, h, spp, acc, flags))
+        alone.append(render_alone(native, ctx, ids[name], p, w, h, spp, flags))
+    segs = native.DeviceArray(8)
+    segs.zero()
+    ctx.render_batch(jobs, segments=segs)
+    for k, (job, ref) in enumerate(zip(JOBS, alone)):
+        got = accs[k].to_numpy(np.float32, ref.size).reshape(-1, 4)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), job
+    assert int(segs.to_numpy(np.uint64, 1)[0]) > 0
+    # and the first job is the oracle's image
+    osc = O.OracleScene(soup["tris"], None, soup["materials"])
+    o, _ = osc.render(with_state(params["soup4"], 0), 64, 48, 3)
+    assert np.array_equal(alone[0].view(np.uint32), o.view(np.uint32))
+
+
+def test_render_batch_continues_each_chain(ctx, native, scenes):
+    """spp = 2 then 3 in two batches equals spp = 5 alone for every image."""
+    ids, params, _, _ = scenes
+    spec = [("soup4", 24, 24, 0, 4), ("ico", 40, 24, 0, 9), ("soup8", 16, 8, 2, 2)]
+    accs = []
+    for name, w, h, flags, rs in spec:
+        acc = native.DeviceArray(16 * w * h)
+        ctx.accum_clear(acc, w * h)
+        accs.append(acc)
+    for spp in (2, 3):
+        ctx.render_batch([(ids[n], with_state(params[n], rs), w, h, spp, accs[k], f)
+                          for k, (n, w, h, f, rs) in enumerate(spec)])
+    for k, (n, w, h, f, rs) in enumerate(spec):
+        ref = render_alone(native, ctx, ids[n], with_state(params[n], rs), w, h, 5, f)
+        got = accs[k].to_numpy(np.float32, ref.size).reshape(-1, 4)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), n
+
+
+def test_render_batch_rejects_overlapping_images(ctx, native, scenes):
+    ids, params, _, _ = scenes
+    acc = native.DeviceArray(16 * 32 * 32)
+    with pytest.raises(native.RTError) as e:
+        ctx.render_batch([(ids["soup4"], params["soup4"], 32, 32, 1, acc),
+                          (ids["soup4"], params["soup4"], 16, 16, 1, acc)])
+    assert e.value.status == 1
+
+
+def _mixed_requests(native, ids, params, rays_by_scene):
+    """Render and ray-query requests over several scenes, sizes and sample counts."""
+    reqs, expect = [], []
+    spec = [("soup4", 64, 48, 6, 0, 1), ("ico", 48, 40, 3, 0, 2), ("soup8", 80, 64, 2, 0, 3),
+            ("ico2", 24, 24, 12, 1, 4), ("soup4", 17, 33, 7, 2, 5), ("ico", 96, 64, 1, 0, 6),
+            ("soup4", 128, 96, 3, 0, 7), ("ico2", 32, 32, 5, 0, 8)]
+    for name, w, h, spp, flags, rs in spec:
+        p = with_state(params[name], rs)
+        out = np.zeros((w * h, 4), np.float32)
+        reqs.append(native.render_request(ids[name], p, w, h, spp, flags | native.REQ_OUT_ACCUM, out=out))
+        expect.append(("render", name, p, w, h, spp, flags, out))
+    for name, (rays, any_hit) in rays_by_scene.items():
+        out = np.zeros(len(rays), L.HIT)
+        reqs.append(native.trace_request(ids[name], rays, any_hit=any_hit, out=out))
+        expect.append(("trace", name, rays, any_hit, out))
+    return reqs, expect
+
+
+def _check_results(native, ctx, ids, expect):
+    for e in expect:
+        if e[0] == "render":
+            _, name, p, w, h, spp, flags, out = e
+            ref = render_alone(native, ctx, ids[name], p, w, h, spp, flags)
+            assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), (name, w, h, spp)
+        else:
+            _, name, rays, any_hit, out = e
+            ref = gpu_trace(ctx, ids[name], rays, any_hit=any_hit)
+            assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), (name, any_hit)
+
+
+def _rays(ids, scenes_data):
+    soup, ico = scenes_data
+    lo, hi = W.scene_bounds(soup["tris"])
+    r1 = W.incoherent_rays(20_000, lo, hi, seed=21)
+    lo, hi = W.scene_bounds(ico["tris"])
+    r2 = W.incoherent_rays(9_001, lo, hi, seed=22)
+    return {"soup4": (r1, False), "ico": (r2, True)}
+
+
+@pytest.mark.parametrize("serial", [False, True])
+def test_server_results_equal_requests_alone(ctx, native, scenes, serial):
+    """A mix of renders and ray queries through a small arena (so requests queue for memory) with short
+    steps (so renders are cut into many sample slices): every result equals the request run alone."""
+    ids, params, soup, ico = scenes
+    reqs, expect = _mixed_requests(native, ids, params, _rays(ids, (soup, ico)))
+    arena = 48 * 20_000 + 2 * 16 * 128 * 96  # the largest query and two of the largest images
+    srv = native.Server(ctx, arena_bytes=arena, max_active=4, step_us=50, pipeline=2, serial=serial)
+    rids = [srv.submit(r) for r in reqs]
+    stats = [srv.wait(rid) for rid in rids]
+    st = srv.stats()
+    srv.close()
+    _check_results(native, ctx, ids, expect)
+    assert st.completed == len(reqs) and st.failed == 0
+    assert st.arena_peak <= st.arena_bytes
+    assert all(s.status == 0 and s.done_ms >= s.admit_ms >= s.submit_ms for s in stats)
+    # admission is in arrival order
+    admits = [s.admit_ms for s in stats]
+    assert admits == sorted(admits)
+    if not serial:
+        assert max(s.steps for s in stats) > 1  # renders were sliced over several steps
+        assert st.steps < sum(s.steps for s in stats)  # and steps held several requests
+    else:
+        assert all(s.steps == 1 for s in stats)
+
+
+def test_server_resolved_output_and_replay(ctx, native, scenes):
+    ids, params, _, _ = scenes
+    w, h, spp = 48, 32, 4
+    reqs, refs = [], []
+    for k in range(12):
+        name = ("soup4", "ico", "soup8")[k % 3]
+        p = with_state(params[name], 100 + k)
+        out = np.zeros((w * h, 4), np.float32)
+        reqs.append(native.render_request(ids[name], p, w, h, spp + k % 3, out=out))
+        acc = native.DeviceArray(16 * w * h)
+        ctx.accum_clear(acc, w * h)
+        ctx.render(ids[name], p, w, h, spp + k % 3, acc)
+        res = native.DeviceArray(16 * w * h)
+        ctx.resolve(acc, w * h, spp + k % 3, res)
+        refs.append((out, res.to_numpy(np.float32, 4 * w * h).reshape(-1, 4)))
+    srv = native.Server(ctx, arena_bytes=64 << 20, step_us=500)
+    arrivals = np.cumsum(np.full(len(reqs), 2.0))  # one request every 2 ms
+    stats = srv.replay(reqs, arrivals)
+    srv.close()
+    for out, ref in refs:
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+    for s, a in zip(stats, arrivals):
+        assert s.status == 0 and s.submit_ms == a and s.done_ms >= s.admit_ms >= s.submit_ms - 1.0
+
+
+def test_server_admission_errors_and_scene_pinning(ctx, native, scenes):
+    ids, params, soup, _ = scenes
+    srv = native.Server(ctx, arena_bytes=1 << 20)
+    too_big = native.render_request(ids["soup4"], params["soup4"], 512, 512, 1)  # 4 MiB of accumulation
+    with pytest.raises(native.RTError) as e:
+        srv.submit(too_big)
+    assert e.value.status == 5  # RT_ERR_BUDGET
+    with pytest.raises(native.RTError) as e:
+        srv.submit(native.render_request(ids["soup4"], params["soup4"], 16, 16, 0))
+    assert e.value.status == 1
+    with pytest.raises(native.RTError) as e:
+        srv.submit(native.render_request(987654, params["soup4"], 16, 16, 1))
+    assert e.value.status == 4
+    # an empty ray query completes at once
+    rid = srv.submit(native.trace_request(ids["soup4"], np.zeros(0, L.RAY)))
+    assert srv.wait(rid).status == 0
+    srv.close()
+    # a scene stays resident while a request that uses it is queued or running
+    extra = ctx.upload(soup["tris"], None, soup["materials"])
+    srv = native.Server(ctx, arena_bytes=64 << 20)
+    rids = [srv.submit(native.render_request(extra, params["soup4"], 256, 256, 64)) for _ in range(3)]
+    with pytest.raises(native.RTError) as e:
+        ctx.release(extra)
+    assert e.value.status == 7  # RT_ERR_BUSY
+    srv.drain()
+    for rid in rids:
+        assert srv.wait(rid).status == 0
+    srv.close()
+    ctx.release(extra)
