@@ -7,7 +7,10 @@ One step = one rt_trace launch over all rays.
 
 Also reported: any-hit rate on the same rays, the algorithmic-bytes roofline of the dominant
 kernel, and a CPU baseline (the oracle's restatement of the reference's own binary-LBVH
-traversal, multithreaded, on a bounded ray sample).
+traversal, multithreaded, on a bounded ray sample); then, as sub-objects of the same JSON line,
+path tracing of the 1M-triangle icosphere at 1080p/16 spp ("pathtrace"), the 256-scene fused
+launch with its build and launch tuners ("batched"), and the serving loop's p50/p99 latency at
+Poisson loads of 0.5x and 0.8x the unbatched capacity ("serving").
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 3]
 """
@@ -150,6 +153,143 @@ def pathtrace_bench(ctx, N, W, L, frames: int, cpu_seconds: float, threads: int)
     return out
 
 
+def batched_bench(ctx, N, W, L, nscenes: int, reps: int):
+    """Many scenes in one launch: 256 clustered soups (triangle counts log-uniform 1e3..2e6, 8 Gaussian
+    clusters each) with incoherent ray batches log-uniform 4,096..1,048,576.  The build tuner picks BVH
+    width and leaf size on 8 sample scenes, the launch tuner picks block/stack/refill for the fused launch;
+    the fused launch is timed against the plain single-scene kernel launched once per scene."""
+    t0 = time.time()
+    work = W.batched_workload(nscenes=nscenes)
+    ntris = np.array([len(t) for t, _, _ in work])
+    nrays = np.array([len(r) for _, _, r in work])
+    log(f"batched: {nscenes} scenes, {ntris.sum()} triangles, {nrays.sum()} rays, generated in "
+        f"{time.time() - t0:.1f} s")
+    d_rays = [N.DeviceArray.from_numpy(r) for _, _, r in work]
+    d_hits = [N.DeviceArray(16 * len(r)) for _, _, r in work]
+    order = np.argsort(ntris, kind="stable")
+    pick = order[np.linspace(0, nscenes - 1, min(8, nscenes)).astype(int)]
+    t0 = time.time()
+    best, cands = ctx.autotune_build([(work[i][0], None, work[i][1], d_rays[i], int(nrays[i]), d_hits[i])
+                                      for i in pick])
+    log(f"build tuner: width {best.width}, leaf {best.leaf_size} ({time.time() - t0:.1f} s); " + ", ".join(
+        f"W{c.width}L{c.leaf_size} {c.trace_ms:.2f} ms" for c in cands))
+    sids = [ctx.upload(t, None, m, width=best.width, leaf_size=best.leaf_size) for t, m, _ in work]
+    items = [(sids[i], int(nrays[i]), d_rays[i], d_hits[i]) for i in range(nscenes)]
+    tb = ctx.autotune_batch(items)
+    log(f"launch tuner: block {tb.cfg.block} lds_stack {tb.cfg.lds_stack} refill {tb.cfg.refill} "
+        f"-> {tb.ms:.2f} ms ({tb.tried} configs)")
+    stream = N.Stream()
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = N.Event(), N.Event()
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            stream.synchronize()
+            ts.append(e0.elapsed_ms(e1))
+        return float(np.median(ts))
+
+    fused_ms = timed(lambda: ctx.trace_batch(items, stream=stream))
+    fused_hits = [h.to_numpy(L.HIT, int(n)) for h, n in zip(d_hits, nrays)]
+    plain_ms = timed(lambda: [ctx.trace(s, r, int(n), h, stream=stream) for (s, n, r, h) in items])
+    same = all(np.array_equal(a.view(np.uint32), h.to_numpy(L.HIT, int(n)).view(np.uint32))
+               for a, h, n in zip(fused_hits, d_hits, nrays))
+    total = int(nrays.sum())
+    out = {"workload": f"{nscenes} scenes, triangle counts log-uniform [1e3, 2e6] (8 Gaussian clusters each, "
+                       f"{int(ntris.sum())} total), ray batches log-uniform [4096, 1048576] ({total} rays), "
+                       "closest-hit, one fused launch",
+           "build": {"width": best.width, "leaf_size": best.leaf_size,
+                     "candidates": [{"width": c.width, "leaf_size": c.leaf_size, "trace_ms": round(c.trace_ms, 3),
+                                     "build_ms": round(c.build_ms, 2)} for c in cands]},
+           "launch": {"block": tb.cfg.block, "lds_stack": tb.cfg.lds_stack, "refill": tb.cfg.refill},
+           "fused_ms": round(fused_ms, 3), "fused_mrays_s": round(total / (fused_ms * 1e-3) / 1e6, 1),
+           "plain_ms": round(plain_ms, 3), "plain_mrays_s": round(total / (plain_ms * 1e-3) / 1e6, 1),
+           "speedup_vs_plain": round(plain_ms / fused_ms, 3), "hits_identical_to_plain": bool(same)}
+    log(f"batched: fused {fused_ms:.2f} ms ({out['fused_mrays_s']} Mrays/s) vs plain per-scene {plain_ms:.2f} ms "
+        f"({out['plain_mrays_s']} Mrays/s), x{out['speedup_vs_plain']}, identical hits {same}")
+    for s in sids:
+        ctx.release(s)
+    return out, best
+
+
+def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int):
+    """The serving loop: 64 resident scenes (triangle counts log-uniform 1e4..4,194,304), render requests
+    (scene U{0..63}, resolution U{256^2, 512^2, 1920x1080}, spp U{1..64}, depth 8, seed = request id)
+    admitted by an arena budget.  The unbatched baseline serves one request at a time with the plain
+    single-scene kernel; its saturated rate is the capacity against which the Poisson loads are set."""
+    t0 = time.time()
+    sizes = W.serving_scene_sizes()
+    sids, params = [], []
+    for k, nt in enumerate(sizes):
+        s = W.served_scene(int(nt), seed=600 + k)
+        sids.append(ctx.upload(s["tris"], None, s["materials"], width=build.width, leaf_size=build.leaf_size))
+        params.append(s["params"])
+    resident = sum(ctx.info(s).device_bytes for s in sids)
+    log(f"serving: {len(sids)} scenes ({int(sizes.sum())} triangles, {resident / 2**30:.2f} GiB resident) in "
+        f"{time.time() - t0:.1f} s")
+    desc = W.serving_requests(nreq, nscenes=len(sids))
+
+    def requests():
+        out = []
+        for d in desc:
+            p = params[d["scene"]].copy()
+            p["randomState"] = d["seed"]
+            out.append(N.render_request(sids[d["scene"]], p, d["width"], d["height"], d["spp"]))
+        return out
+
+    samples = np.array([d["width"] * d["height"] * d["spp"] for d in desc], dtype=np.float64)
+    arena = int(arena_gib * 2**30)
+
+    def run(serial: bool, arrivals):
+        srv = N.Server(ctx, arena_bytes=arena, max_active=64, step_us=step_us, serial=serial)
+        warm = requests()[:4]
+        srv.replay(warm, np.zeros(len(warm)))
+        before = srv.stats()
+        st = srv.replay(requests(), arrivals)
+        after = srv.stats()
+        srv.close()
+        lat = np.array([s.done_ms - s.submit_ms for s in st])
+        makespan = max(s.done_ms for s in st) - float(arrivals[0])
+        segs = after.segments - before.segments
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "mean_ms": round(float(lat.mean()), 2), "makespan_s": round(makespan / 1e3, 3),
+                "requests_per_s": round(len(st) / (makespan / 1e3), 2),
+                "msamples_per_s": round(samples.sum() / (makespan / 1e3) / 1e6, 1),
+                "msegments_per_s": round(segs / (makespan / 1e3) / 1e6, 1),
+                "device_busy_frac": round((after.busy_ms - before.busy_ms) / makespan, 3),
+                "steps": int(after.steps - before.steps), "arena_peak_bytes": int(after.arena_peak)}
+
+    zero = np.zeros(nreq)
+    serial_sat = run(True, zero)
+    cap = serial_sat["requests_per_s"]
+    log(f"serving: unbatched capacity {cap} req/s ({serial_sat['msegments_per_s']} Msegments/s)")
+    batched_sat = run(False, zero)
+    log(f"serving: batched saturated {batched_sat['requests_per_s']} req/s "
+        f"({batched_sat['msegments_per_s']} Msegments/s)")
+    loads = {}
+    for frac in (0.5, 0.8):
+        arr = W.poisson_arrivals(nreq, frac * cap, seed=8)
+        loads[str(frac)] = {"qps": round(frac * cap, 3), "batched": run(False, arr), "unbatched": run(True, arr)}
+        log(f"serving at {frac}x capacity ({frac * cap:.2f} req/s): batched p50 {loads[str(frac)]['batched']['p50_ms']} "
+            f"p99 {loads[str(frac)]['batched']['p99_ms']} ms; unbatched p50 {loads[str(frac)]['unbatched']['p50_ms']} "
+            f"p99 {loads[str(frac)]['unbatched']['p99_ms']} ms")
+    for s in sids:
+        ctx.release(s)
+    return {"workload": f"{len(sids)} resident scenes ({int(sizes.sum())} triangles, log-uniform 1e4..4194304), "
+                        f"{nreq} render requests: resolution U{{256^2, 512^2, 1920x1080}}, spp U{{1..64}}, depth 8, "
+                        f"seed = request id; Poisson arrivals",
+            "arena_bytes": arena, "step_us": step_us, "resident_bytes": int(resident),
+            "unbatched_capacity_rps": cap, "saturated": {"batched": batched_sat, "unbatched": serial_sat},
+            # the same requests either way, so pixel-samples per second compare the work done (traced
+            # segments do not: the unbatched path keeps every primary hit in registers for all samples)
+            "throughput_ratio_saturated": round(batched_sat["msamples_per_s"] / max(serial_sat["msamples_per_s"], 1e-9),
+                                                3),
+            "poisson": loads}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +308,12 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="only run the PMC calibration copies")
     ap.add_argument("--no-pathtrace", action="store_true", help="skip the path-tracing measurement")
     ap.add_argument("--pt-frames", type=int, default=5)
+    ap.add_argument("--no-batched", action="store_true", help="skip the many-scene fused-launch measurement")
+    ap.add_argument("--batched-scenes", type=int, default=256)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-requests", type=int, default=160)
+    ap.add_argument("--serve-arena-gib", type=float, default=2.0)
+    ap.add_argument("--serve-step-us", type=int, default=4000)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -307,6 +453,12 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
         out["pathtrace"] = pathtrace_bench(ctx, N, W, L, args.pt_frames,
                                            0.0 if args.no_cpu_baseline else args.cpu_seconds / 2, threads)
+    build = N.BuildOptions(0, inf.width, inf.leaf_size, 0)
+    if not args.no_batched:
+        out["batched"], build = batched_bench(ctx, N, W, L, args.batched_scenes, 5)
+    if not args.no_serving:
+        out["serving"] = serving_bench(ctx, N, W, L, args.serve_requests, build, args.serve_arena_gib,
+                                       args.serve_step_us)
     print(json.dumps(out), flush=True)
 
 

@@ -226,6 +226,43 @@ typedef struct rt_tune_result {
 int rt_autotune(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n, rt_hit *hits,
                 uint32_t flags, rt_tune_result *out);
 
+/* Times launch configurations (block size, LDS stack depth, refill threshold) of the fused
+ * multi-scene launch on these items and makes the best the context's default for
+ * rt_trace_batch calls without a cfg.  The items' hits are overwritten. */
+int rt_autotune_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems, uint32_t flags,
+                      rt_tune_result *out);
+
+/* A sample scene for the build-parameter tuner: host geometry (as for rt_scene_upload,
+ * world space) and device rays to trace against it. */
+typedef struct rt_tune_scene {
+    const rt_triangle *tris;
+    uint32_t ntri;
+    uint32_t nsph;
+    const rt_sphere *spheres;
+    const rt_material *materials;
+    uint32_t nmat;
+    uint32_t nrays;
+    const rt_ray *rays; /* device */
+    rt_hit *hits;       /* device, overwritten */
+} rt_tune_scene;
+
+typedef struct rt_build_candidate {
+    uint32_t width, leaf_size;
+    float build_ms;         /* device build time summed over the sample scenes */
+    float trace_ms;         /* one fused launch over all sample rays, best launch configuration */
+    rt_launch_config cfg;   /* that configuration */
+    uint64_t device_bytes;  /* resident size of the sample scenes */
+} rt_build_candidate;
+
+/* Builds the sample scenes with every BVH width in {2, 4, 8} and leaf size in {1, 2, 4},
+ * times a fused launch over all their rays for each (searching launch configurations),
+ * and reports every candidate (up to `capacity`, fastest first) and the best build options.
+ * The tuner for RaytracerBVH's fixed binary LBVH (RaytracerBVH.cpp:734-997), which has no
+ * such parameters.  flags: RT_TRACE_ANY tunes any-hit. */
+int rt_autotune_build(rt_context *ctx, const rt_tune_scene *scenes, uint32_t nscenes, uint32_t flags,
+                      rt_build_candidate *results, uint32_t capacity, uint32_t *ncandidates,
+                      rt_build_options *best);
+
 /* ---------------- serving loop ----------------
  * No counterpart in the reference, which renders one scene per frame on one thread
  * (RaytracerBVH.hpp:517-571).  A server keeps scenes resident (uploaded to its context),
@@ -252,6 +289,8 @@ typedef struct rt_server_config {
     uint32_t step_us;            /* target GPU time of one scheduling step (0 = 4000) */
     uint32_t pipeline;           /* steps enqueued ahead of the device (0 = 2) */
     uint32_t flags;              /* RT_SERVER_* */
+    uint32_t max_spp_step;       /* samples per pixel a render advances per step at most (0 = 4) */
+    uint32_t _reserved;
     rt_launch_config render_cfg; /* zeros = defaults */
     rt_launch_config trace_cfg;
 } rt_server_config;

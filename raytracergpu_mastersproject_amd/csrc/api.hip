@@ -30,28 +30,6 @@ static_assert(sizeof(LeafRecord) == 64, "leaf record");
 
 namespace {
 
-template <class F> int guarded(rt_context *ctx, F &&f) {
-    std::unique_lock<std::recursive_mutex> lk;
-    if (ctx) lk = std::unique_lock<std::recursive_mutex>(ctx->mu);
-    try {
-        if (ctx) {
-            hipError_t e = hipSetDevice(ctx->device);
-            if (e != hipSuccess) throw Error(RT_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
-        }
-        f();
-        return RT_OK;
-    } catch (const Error &e) {
-        if (ctx) ctx->last_error = e.what();
-        return e.status;
-    } catch (const std::bad_alloc &) {
-        if (ctx) ctx->last_error = "host allocation failed";
-        return RT_ERR_NO_MEMORY;
-    } catch (const std::exception &e) {
-        if (ctx) ctx->last_error = e.what();
-        return RT_ERR_INVALID;
-    }
-}
-
 int hip_status(hipError_t e) {
     if (e == hipSuccess) return RT_OK;
     return e == hipErrorOutOfMemory ? RT_ERR_NO_MEMORY : RT_ERR_HIP;
@@ -234,64 +212,10 @@ int rt_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems,
     return guarded(ctx, [&] {
         RT_REQUIRE((flags & ~RT_TRACE_ANY) == 0, "unknown trace flags");
         LaunchCfg c;
-        if (nitems) c = resolve_cfg(ctx, ctx->scene(items[0].scene_id), cfg, (flags & RT_TRACE_ANY) != 0);
-        launch_trace_batch(ctx, items, nitems, flags, c, static_cast<hipStream_t>(stream));
-    });
-}
-
-int rt_autotune(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n, rt_hit *hits, uint32_t flags,
-                rt_tune_result *out) {
-    if (!ctx) return RT_ERR_INVALID;
-    return guarded(ctx, [&] {
-        RT_REQUIRE(n > 0, "autotune needs rays");
-        Scene &s = ctx->scene(scene_id);
         const bool any = (flags & RT_TRACE_ANY) != 0;
-        std::vector<LaunchCfg> cands;
-        for (uint32_t block : {64u, 128u, 256u})
-            for (uint32_t st : {4u, 8u, 16u}) {
-                if (s.stack_bound > st + 184u) continue;
-                cands.push_back(LaunchCfg{block, st, 0u, 16u});
-                for (uint32_t refill : {8u, 16u, 32u}) cands.push_back(LaunchCfg{block, st, 1u, refill});
-            }
-        RT_REQUIRE(!cands.empty(), "no launch configuration fits the tree's stack bound");
-        hipStream_t stream;
-        RT_HIP(hipStreamCreate(&stream));
-        hipEvent_t e0, e1;
-        RT_HIP(hipEventCreate(&e0));
-        RT_HIP(hipEventCreate(&e1));
-        float best = 1e30f;
-        LaunchCfg bestc = cands[0];
-        for (const LaunchCfg &c : cands) {
-            launch_trace(ctx, s, rays, n, hits, nullptr, flags, c, stream);  // warm-up
-            float times[3];
-            for (float &tm : times) {
-                RT_HIP(hipEventRecord(e0, stream));
-                launch_trace(ctx, s, rays, n, hits, nullptr, flags, c, stream);
-                RT_HIP(hipEventRecord(e1, stream));
-                RT_HIP(hipEventSynchronize(e1));
-                RT_HIP(hipEventElapsedTime(&tm, e0, e1));
-            }
-            std::sort(times, times + 3);
-            if (times[1] < best) {
-                best = times[1];
-                bestc = c;
-            }
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipStreamDestroy(stream);
-        if (!s.has_tuned) {
-            s.tuned_closest = LaunchCfg{};
-            s.tuned_any = LaunchCfg{};
-        }
-        (any ? s.tuned_any : s.tuned_closest) = bestc;
-        s.has_tuned = true;
-        if (out) {
-            out->cfg = rt_launch_config{bestc.block, bestc.lds_stack, bestc.persistent, bestc.refill};
-            out->ms = best;
-            out->rays_per_s = best > 0.0f ? (float)(n / (best * 1e-3)) : 0.0f;
-            out->tried = (uint32_t)cands.size();
-        }
+        if (nitems) c = resolve_cfg(ctx, ctx->scene(items[0].scene_id), cfg, any);
+        if (!cfg && ctx->has_batch_tuned) c = any ? ctx->batch_any : ctx->batch_closest;
+        launch_trace_batch(ctx, items, nitems, flags, c, static_cast<hipStream_t>(stream));
     });
 }
 

@@ -167,6 +167,7 @@ __device__ __forceinline__ void shade(const RenderArgs &a, PixelState &ps, Trav 
         ps.pc_slot = ps.hslot;
         ps.pc_t = ps.ht;
         ps.pc_valid = true;
+        if (J.pcache) J.pcache[ps.pix] = make_uint2(ps.hslot, __float_as_uint(ps.ht));  // for later launches
     }
     if (ps.hslot == kEmpty) {
         const v3 background{0.0f, 0.0f, 0.0f};  // _BACKGROUND_COLOR
@@ -299,6 +300,12 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
                     ps.seed0 = rng_seed(x, y, J.random_state);
                     ps.sample = 0;
                     ps.pc_valid = false;
+                    if (J.pcache && J.pcache_valid) {  // traced by an earlier launch over this image
+                        const uint2 c = J.pcache[ps.pix];
+                        ps.pc_slot = c.x;
+                        ps.pc_t = __uint_as_float(c.y);
+                        ps.pc_valid = true;
+                    }
                     if (J.spp == 0) ps.phase = kIdle;
                     else begin_sample<MULTI>(a, ps, t, segs);
                 }
@@ -419,6 +426,8 @@ RenderJob make_render_job(const Scene &sc, const rt_params &p, uint32_t width, u
     j.tiles_x = (width + 7) / 8;
     j.first_item = 0;
     j.nitems = j.tiles_x * ((height + 7) / 8) * 64u;
+    j.pcache = nullptr;
+    j.pcache_valid = 0;
     camera_setup(p, width, height, j.cam);
     return j;
 }

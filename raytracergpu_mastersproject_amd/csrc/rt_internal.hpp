@@ -103,9 +103,10 @@ struct RenderJob {
     const uint32_t *slot_mat;
     const rt_material *mats;
     float *accum;
+    uint2 *pcache;  // optional per-pixel first-segment hit {slot, t bits} kept between launches
     uint32_t nwide, width, height, spp;
     uint32_t depth, random_state, flags, tiles_x;
-    uint32_t first_item, nitems, _pad0, _pad1;
+    uint32_t first_item, nitems, pcache_valid, _pad0;
     float cam[12];  // pixel00, deltaU, deltaV, origin
 };
 constexpr uint32_t kRenderChunk = 64;  // work items (one 8x8 tile) per atomic on the render work counter
@@ -182,6 +183,9 @@ struct rt_context {
     uint32_t next_counter = 0;
     static constexpr uint32_t kCounterSlots = 4096;
     rtgpu::TableRing tables;
+    // launch configuration chosen by rt_autotune_batch for fused multi-scene launches
+    bool has_batch_tuned = false;
+    rtgpu::LaunchCfg batch_closest, batch_any;
     // Held by every entry point for its duration and by a server's worker thread while
     // it enqueues a step, so a context may be shared by the caller and its servers.
     std::recursive_mutex mu;
@@ -198,6 +202,30 @@ struct rt_context {
 };
 
 namespace rtgpu {
+
+// Runs f at the ABI boundary: holds the context lock, selects its device, and turns
+// exceptions into a status code with the message kept for rt_last_error.
+template <class F> inline int guarded(rt_context *ctx, F &&f) {
+    std::unique_lock<std::recursive_mutex> lk;
+    if (ctx) lk = std::unique_lock<std::recursive_mutex>(ctx->mu);
+    try {
+        if (ctx) {
+            hipError_t e = hipSetDevice(ctx->device);
+            if (e != hipSuccess) throw Error(RT_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+        }
+        f();
+        return RT_OK;
+    } catch (const Error &e) {
+        if (ctx) ctx->last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return RT_ERR_NO_MEMORY;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->last_error = e.what();
+        return RT_ERR_INVALID;
+    }
+}
 
 // build.hip
 void build_scene(rt_context *ctx, Scene &sc, const rt_triangle *tris, uint32_t ntri, const rt_sphere *sph,

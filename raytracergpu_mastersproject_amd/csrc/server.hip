@@ -140,8 +140,13 @@ struct rt_server {
 namespace rtgpu {
 namespace {
 
+// A render holds its accumulation image and, unless primary caching is off, the
+// per-pixel first-segment hits that later slices reuse; a ray query holds rays + hits.
+bool keeps_primaries(const rt_request &r) { return !(r.render_flags & RT_RENDER_NO_PRIMARY_CACHE); }
+
 uint64_t request_bytes(const rt_request &r) {
-    if (r.kind == RT_REQ_RENDER) return Arena::round(16ull * r.width * r.height);
+    const uint64_t npix = (uint64_t)r.width * r.height;
+    if (r.kind == RT_REQ_RENDER) return Arena::round(16ull * npix) + (keeps_primaries(r) ? Arena::round(8ull * npix) : 0);
     return Arena::round(32ull * r.nrays) + Arena::round(16ull * r.nrays);
 }
 
@@ -205,6 +210,9 @@ bool enqueue_step(rt_server *s) {
     std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
     RT_HIP(hipEventRecord(step.start, s->stream));
     // samples per render this step: an equal share of the step's budget, at least one pass
+    // and at most max_spp_step passes.  A lane runs its pixel's samples of a slice one after
+    // another, so the cap bounds how long the slowest pixel keeps a step's launch alive
+    // after the rest of the device has run dry.
     const double budget_samples = s->sample_rate * s->cfg.step_us;
     const double share = budget_samples / (double)std::max<size_t>(1, renders.size());
     std::map<uint32_t, std::vector<RenderJob>> by_width;
@@ -213,7 +221,8 @@ bool enqueue_step(rt_server *s) {
     for (auto &r : renders) {
         const uint64_t npix = (uint64_t)r->req.width * r->req.height;
         uint32_t q = serial ? r->remaining
-                            : (uint32_t)std::min<double>(r->remaining, std::max(1.0, std::floor(share / (double)npix)));
+                            : (uint32_t)std::min<double>({(double)r->remaining, (double)s->cfg.max_spp_step,
+                                                          std::max(1.0, std::floor(share / (double)npix))});
         q = std::max(1u, std::min(q, r->remaining));
         float *accum = reinterpret_cast<float *>(arena_ptr(s, r->offset));
         const uint32_t kflags = r->req.render_flags & ~RT_REQ_OUT_ACCUM;
@@ -227,8 +236,12 @@ bool enqueue_step(rt_server *s) {
             launch_render(s->ctx, *r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum,
                           s->segments.as<unsigned long long>(), rc, s->stream);
         } else {
-            by_width[r->scene->width].push_back(
-                make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum));
+            RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
+            if (keeps_primaries(r->req)) {
+                j.pcache = reinterpret_cast<uint2 *>(arena_ptr(s, r->offset + Arena::round(16ull * npix)));
+                j.pcache_valid = r->done_units > 0 ? 1u : 0u;  // an earlier slice has stored them
+            }
+            by_width[r->scene->width].push_back(j);
         }
         r->remaining -= q;
         r->done_units += q;
@@ -391,6 +404,7 @@ int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **o
         if (!s->cfg.step_us) s->cfg.step_us = 4000;
         if (!s->cfg.pipeline) s->cfg.pipeline = 2;
         s->cfg.pipeline = std::min(s->cfg.pipeline, 8u);
+        if (!s->cfg.max_spp_step) s->cfg.max_spp_step = 4;
         RT_REQUIRE((s->cfg.flags & ~RT_SERVER_SERIAL) == 0, "unknown server flags");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         RT_HIP(hipSetDevice(ctx->device));

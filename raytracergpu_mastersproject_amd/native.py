@@ -58,6 +58,17 @@ class TuneResult(C.Structure):
                 ("_pad", C.c_uint32)]
 
 
+class TuneScene(C.Structure):
+    _fields_ = [("tris", C.c_void_p), ("ntri", C.c_uint32), ("nsph", C.c_uint32), ("spheres", C.c_void_p),
+                ("materials", C.c_void_p), ("nmat", C.c_uint32), ("nrays", C.c_uint32), ("rays", C.c_void_p),
+                ("hits", C.c_void_p)]
+
+
+class BuildCandidate(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("leaf_size", C.c_uint32), ("build_ms", C.c_float), ("trace_ms", C.c_float),
+                ("cfg", LaunchConfig), ("device_bytes", C.c_uint64)]
+
+
 class RenderJob(C.Structure):
     _fields_ = [("scene_id", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("flags", C.c_uint32), ("_reserved", C.c_uint32), ("accum", C.c_void_p), ("params", C.c_ubyte * 80)]
@@ -71,7 +82,8 @@ SERVER_SERIAL = 1
 
 class ServerConfig(C.Structure):
     _fields_ = [("arena_bytes", C.c_uint64), ("max_active", C.c_uint32), ("step_us", C.c_uint32),
-                ("pipeline", C.c_uint32), ("flags", C.c_uint32), ("render_cfg", LaunchConfig),
+                ("pipeline", C.c_uint32), ("flags", C.c_uint32), ("max_spp_step", C.c_uint32),
+                ("_reserved", C.c_uint32), ("render_cfg", LaunchConfig),
                 ("trace_cfg", LaunchConfig)]
 
 
@@ -111,6 +123,9 @@ _SIGS = {
     "rt_resolve": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp]),
     "rt_trace_batch": (_i32, [_vp, C.POINTER(BatchItem), _u32, _u32, C.POINTER(LaunchConfig), _vp]),
     "rt_autotune": (_i32, [_vp, _u32, _vp, _u32, _vp, _u32, C.POINTER(TuneResult)]),
+    "rt_autotune_batch": (_i32, [_vp, C.POINTER(BatchItem), _u32, _u32, C.POINTER(TuneResult)]),
+    "rt_autotune_build": (_i32, [_vp, C.POINTER(TuneScene), _u32, _u32, C.POINTER(BuildCandidate), _u32,
+                                 C.POINTER(_u32), C.POINTER(BuildOptions)]),
     "rt_render_batch": (_i32, [_vp, C.POINTER(RenderJob), _u32, _vp, C.POINTER(LaunchConfig), _vp]),
     "rt_server_create": (_i32, [_vp, C.POINTER(ServerConfig), C.POINTER(_vp)]),
     "rt_server_destroy": (_i32, [_vp]),
@@ -362,6 +377,31 @@ class Context:
         self._ck(lib.rt_autotune(self.h, sid, _ptr(rays), n, _ptr(hits), TRACE_ANY if any_hit else 0, C.byref(res)))
         return res
 
+    def autotune_batch(self, items, any_hit=False) -> TuneResult:
+        """items: (scene_id, nrays, rays, hits) tuples; the best shape becomes the default for trace_batch."""
+        arr = (BatchItem * len(items))(*[BatchItem(s, n, _ptr(r), _ptr(h)) for (s, n, r, h) in items])
+        res = TuneResult()
+        self._ck(lib.rt_autotune_batch(self.h, arr, len(items), TRACE_ANY if any_hit else 0, C.byref(res)))
+        return res
+
+    def autotune_build(self, samples, any_hit=False):
+        """samples: (tris, spheres, materials, rays DeviceArray, nrays, hits DeviceArray) tuples.
+        Returns (best BuildOptions, [BuildCandidate fastest first])."""
+        keep, arr = [], (TuneScene * len(samples))()
+        for k, (tris, sph, mats, rays, n, hits) in enumerate(samples):
+            tris = np.ascontiguousarray(tris, dtype=L.TRIANGLE)
+            sph = np.ascontiguousarray(sph if sph is not None else np.zeros(0, L.SPHERE), dtype=L.SPHERE)
+            mats = np.ascontiguousarray(mats, dtype=L.MATERIAL)
+            keep += [tris, sph, mats]
+            arr[k] = TuneScene(_ptr(tris), len(tris), len(sph), _ptr(sph), _ptr(mats), len(mats), n, _ptr(rays),
+                               _ptr(hits))
+        cands = (BuildCandidate * 16)()
+        ncand = C.c_uint32()
+        best = BuildOptions()
+        self._ck(lib.rt_autotune_build(self.h, arr, len(samples), TRACE_ANY if any_hit else 0, cands, 16,
+                                       C.byref(ncand), C.byref(best)))
+        return best, list(cands[:min(ncand.value, 16)])
+
     # ---- path tracing ----
     def accum_clear(self, accum: DeviceArray, npix: int, stream=None):
         self._ck(lib.rt_accum_clear(self.h, _ptr(accum), npix, stream.handle if stream else None))
@@ -415,8 +455,8 @@ class Server:
     """A serving loop over a context's resident scenes (rt_server_*)."""
 
     def __init__(self, ctx: Context, arena_bytes=1 << 30, max_active=64, step_us=4000, pipeline=2, serial=False,
-                 render_cfg: LaunchConfig | None = None, trace_cfg: LaunchConfig | None = None):
-        cfg = ServerConfig(arena_bytes, max_active, step_us, pipeline, SERVER_SERIAL if serial else 0)
+                 max_spp_step=0, render_cfg: LaunchConfig | None = None, trace_cfg: LaunchConfig | None = None):
+        cfg = ServerConfig(arena_bytes, max_active, step_us, pipeline, SERVER_SERIAL if serial else 0, max_spp_step)
         if render_cfg:
             cfg.render_cfg = render_cfg
         if trace_cfg:

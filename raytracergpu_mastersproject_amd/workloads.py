@@ -280,3 +280,60 @@ def batched_workload(nscenes=256, tri_lo=1_000, tri_hi=2_000_000, ray_lo=4096, r
         rays = incoherent_rays(max(1, int(nrays[k])), lo, hi, seed=100000 + k)
         out.append((tris, mats, rays))
     return out
+
+
+def material_palette(seed: int = 6) -> np.ndarray:
+    """40 materials: 24 lambertian (albedo U[0.2,0.9]^3), 12 metal (albedo U[0.5,1]^3, fuzz U[0,0.3]),
+    4 dielectric (ior 1.5), then one emitter (albedo 4) at index 40.  A triangle drawing its index
+    uniformly from 0..39 is 60% lambertian, 30% metal, 10% dielectric."""
+    rng = SplitMix64(seed)
+    col = rng.uniform(3 * 40).reshape(40, 3)
+    fuzz = rng.uniform(40, 0.0, 0.3)
+    m = np.concatenate([make_materials(0.2 + 0.7 * col[:24], L.DIFFUSE),
+                        make_materials(0.5 + 0.5 * col[24:36], L.METALLIC, fuzz[24:36]),
+                        make_materials(np.ones((4, 3)), L.DIELECTRIC, np.full(4, 1.5)),
+                        make_materials([4.0, 4.0, 4.0], L.LIGHT)])
+    return m
+
+
+def served_scene(ntri: int, seed: int, nclusters: int = 8) -> RenderSetup:
+    """A clustered soup of about `ntri` triangles (8 Gaussian clusters, sigma 0.05, centres U[-1,1]^3)
+    with palette materials, a lambertian ground quad (y = -1.1), an emissive quad (y = 3), and the
+    soup scene's camera ((0,0.5,-3.5) -> origin, vfov 40)."""
+    per = max(1, ntri // nclusters)
+    tris, _ = cluster_scene(nclusters, per, seed=seed)
+    tris["materialIndex"] = (SplitMix64(seed ^ 0x5EED).u64(len(tris)) % np.uint64(40)).astype(np.uint32)
+    mats = material_palette()
+    ground = quad(-1.1, -3.0, 3.0, -3.0, 3.0, 0)
+    light = quad(3.0, -1.5, 1.5, -1.5, 1.5, 40, flip=True)
+    tris = np.concatenate([tris, ground, light])
+    params = L.make_params((0.0, 0.5, -3.5), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, len(tris), 0, len(mats), 8, 0)
+    return RenderSetup(tris=tris, spheres=np.zeros(0, L.SPHERE), materials=mats, params=params, width=0, height=0,
+                       spp=0)
+
+
+def serving_scene_sizes(nscenes: int = 64, lo: float = 1e4, hi: float = 4_194_304, seed: int = 6) -> np.ndarray:
+    """Triangle counts log-uniform in [lo, hi]."""
+    return np.exp(SplitMix64(seed).uniform(nscenes, np.log(lo), np.log(hi))).astype(np.int64)
+
+
+SERVING_RESOLUTIONS = ((256, 256), (512, 512), (1920, 1080))
+
+
+def serving_requests(n: int, nscenes: int = 64, max_spp: int = 64, seed: int = 7, scale: float = 1.0):
+    """Request descriptors (scene, width, height, spp, seed = request index): scene U{0..nscenes-1},
+    resolution U{256^2, 512^2, 1920x1080} (scaled by `scale` for tests), spp U{1..max_spp}, depth 8."""
+    rng = SplitMix64(seed)
+    u = rng.u64(3 * n).reshape(n, 3)
+    out = []
+    for i in range(n):
+        w, h = SERVING_RESOLUTIONS[int(u[i, 1] % np.uint64(3))]
+        out.append({"scene": int(u[i, 0] % np.uint64(nscenes)), "width": max(8, int(w * scale)),
+                    "height": max(8, int(h * scale)), "spp": 1 + int(u[i, 2] % np.uint64(max_spp)), "seed": i})
+    return out
+
+
+def poisson_arrivals(n: int, rate_per_s: float, seed: int = 8) -> np.ndarray:
+    """Arrival times (ms) of a Poisson process of the given rate."""
+    gaps = -np.log1p(-SplitMix64(seed).uniform(n)) / rate_per_s
+    return np.cumsum(gaps) * 1000.0

@@ -124,7 +124,7 @@ def test_ctypes_structs_match_the_c_header():
     structs = {"rt_build_options": N.BuildOptions, "rt_launch_config": N.LaunchConfig, "rt_scene_info": N.SceneInfo,
                "rt_batch_item": N.BatchItem, "rt_tune_result": N.TuneResult, "rt_render_job": N.RenderJob,
                "rt_server_config": N.ServerConfig, "rt_request": N.Request, "rt_request_stats": N.RequestStats,
-               "rt_server_stats": N.ServerStats}
+               "rt_server_stats": N.ServerStats, "rt_tune_scene": N.TuneScene, "rt_build_candidate": N.BuildCandidate}
     lines = []
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

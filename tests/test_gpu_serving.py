@@ -218,3 +218,61 @@ def test_server_admission_errors_and_scene_pinning(ctx, native, scenes):
         assert srv.wait(rid).status == 0
     srv.close()
     ctx.release(extra)
+
+
+def test_build_and_launch_tuners(ctx, native, O):
+    """The build tuner reports all nine (width, leaf) candidates fastest first; scenes built with its choice
+    and traced in one fused launch with the launch tuner's shape give the oracle's hits."""
+    samples, keep = [], []
+    for k, (nc, per) in enumerate([(4, 3000), (8, 500), (2, 9000)]):
+        tris, mats = W.cluster_scene(nc, per, seed=70 + k)
+        lo, hi = W.scene_bounds(tris)
+        rays = W.incoherent_rays(4096 * (k + 1), lo, hi, seed=80 + k)
+        dr, dh = native.DeviceArray.from_numpy(rays), native.DeviceArray(16 * len(rays))
+        samples.append((tris, None, mats, dr, len(rays), dh))
+        keep.append((tris, mats, rays))
+    best, cands = ctx.autotune_build(samples)
+    assert len(cands) == 9
+    assert sorted((c.width, c.leaf_size) for c in cands) == [(w, l) for w in (2, 4, 8) for l in (1, 2, 4)]
+    assert [c.trace_ms for c in cands] == sorted(c.trace_ms for c in cands)
+    assert (best.width, best.leaf_size) == (cands[0].width, cands[0].leaf_size)
+    sids = [ctx.upload(t, None, m, width=best.width, leaf_size=best.leaf_size) for t, m, _ in keep]
+    items = [(sid, s[4], s[3], s[5]) for sid, s in zip(sids, samples)]
+    tb = ctx.autotune_batch(items)
+    assert tb.cfg.block in (64, 128, 256) and tb.cfg.lds_stack in (4, 8, 16) and tb.tried > 0
+    ctx.trace_batch(items)  # the tuned shape is now the default
+    for (tris, mats, rays), s in zip(keep, samples):
+        got = s[5].to_numpy(L.HIT, len(rays))
+        ref = O.OracleScene(tris, None, mats).trace(rays[:2048], O.BRUTE)
+        assert np.array_equal(got["prim"][:2048], ref["prim"])
+        assert np.array_equal(got["t"][:2048].view(np.uint32), ref["t"].view(np.uint32))
+    for sid in sids:
+        ctx.release(sid)
+
+
+def test_served_scene_requests_through_the_server(ctx, native):
+    """The serving benchmark's own scenes and request mix (scaled down): served results equal renders alone."""
+    sids, params = [], []
+    for k, nt in enumerate((12_000, 30_000)):
+        s = W.served_scene(nt, seed=600 + k)
+        sids.append(ctx.upload(s["tris"], None, s["materials"]))
+        params.append(s["params"])
+    desc = W.serving_requests(10, nscenes=2, max_spp=6, scale=1 / 16)
+    reqs, outs = [], []
+    for d in desc:
+        p = params[d["scene"]].copy()
+        p["randomState"] = d["seed"]
+        out = np.zeros((d["width"] * d["height"], 4), np.float32)
+        reqs.append(native.render_request(sids[d["scene"]], p, d["width"], d["height"], d["spp"],
+                                          native.REQ_OUT_ACCUM, out=out))
+        outs.append((sids[d["scene"]], p, d, out))
+    srv = native.Server(ctx, arena_bytes=8 << 20, step_us=300)
+    stats = srv.replay(reqs, W.poisson_arrivals(len(reqs), 500.0))
+    srv.close()
+    assert all(s.status == 0 for s in stats)
+    for sid, p, d, out in outs:
+        ref = render_alone(native, ctx, sid, p, d["width"], d["height"], d["spp"])
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), d
+    assert any(o[3][:, :3].max() > 0 for o in outs)  # the emitter is reached
+    for sid in sids:
+        ctx.release(sid)
