@@ -215,7 +215,7 @@ def batched_bench(ctx, N, W, L, nscenes: int, reps: int):
     return out, best
 
 
-def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int):
+def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int, max_spp_step: int = 0):
     """The serving loop: 64 resident scenes (triangle counts log-uniform 1e4..4,194,304), render requests
     (scene U{0..63}, resolution U{256^2, 512^2, 1920x1080}, spp U{1..64}, depth 8, seed = request id)
     admitted by an arena budget.  The unbatched baseline serves one request at a time with the plain
@@ -244,7 +244,8 @@ def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int
     arena = int(arena_gib * 2**30)
 
     def run(serial: bool, arrivals):
-        srv = N.Server(ctx, arena_bytes=arena, max_active=64, step_us=step_us, serial=serial)
+        srv = N.Server(ctx, arena_bytes=arena, max_active=64, step_us=step_us, serial=serial,
+                       max_spp_step=max_spp_step)
         warm = requests()[:4]
         srv.replay(warm, np.zeros(len(warm)))
         before = srv.stats()
@@ -313,7 +314,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-arena-gib", type=float, default=2.0)
-    ap.add_argument("--serve-step-us", type=int, default=4000)
+    ap.add_argument("--serve-step-us", type=int, default=8000)
+    ap.add_argument("--serve-max-spp-step", type=int, default=0)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -458,7 +460,7 @@ def main():
         out["batched"], build = batched_bench(ctx, N, W, L, args.batched_scenes, 5)
     if not args.no_serving:
         out["serving"] = serving_bench(ctx, N, W, L, args.serve_requests, build, args.serve_arena_gib,
-                                       args.serve_step_us)
+                                       args.serve_step_us, args.serve_max_spp_step)
     print(json.dumps(out), flush=True)
 
 

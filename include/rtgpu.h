@@ -165,6 +165,12 @@ int rt_trace(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n,
 
 #define RT_RENDER_REFERENCE_MATERIALS 1u /* raytraceBVH.comp:100-115: METALLIC/DIELECTRIC absorb */
 #define RT_RENDER_NO_PRIMARY_CACHE 2u    /* trace the (identical) primary segment of every sample */
+/* How a pixel's samples are spread over lanes.  By default an image with fewer pixels than
+ * about four per lane of the device runs every (pixel, sample) on its own lane and sums the
+ * samples in order afterwards; larger images run each pixel's samples on one lane.  The
+ * image is the same either way. */
+#define RT_RENDER_SPLIT_SAMPLES 4u       /* always one lane per (pixel, sample) */
+#define RT_RENDER_NO_SPLIT 8u            /* always one lane per pixel */
 
 /* Clear an RGBA32F accumulation image to (0,0,0,1) (RaytracerBVH.cpp:750-776). */
 int rt_accum_clear(rt_context *ctx, float *accum, uint32_t npix, void *stream);
@@ -286,7 +292,7 @@ typedef struct rt_server rt_server;
 typedef struct rt_server_config {
     uint64_t arena_bytes;        /* device memory for request buffers: the admission budget (0 = 4 GiB) */
     uint32_t max_active;         /* requests admitted at once (0 = 64) */
-    uint32_t step_us;            /* target GPU time of one scheduling step (0 = 4000) */
+    uint32_t step_us;            /* target GPU time of one scheduling step (0 = 8000) */
     uint32_t pipeline;           /* steps enqueued ahead of the device (0 = 2) */
     uint32_t flags;              /* RT_SERVER_* */
     uint32_t max_spp_step;       /* samples per pixel a render advances per step at most (0 = 4) */

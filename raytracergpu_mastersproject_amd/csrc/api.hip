@@ -165,8 +165,7 @@ int rt_render(rt_context *ctx, uint32_t scene_id, const rt_params *params, uint3
               void *stream) {
     if (!ctx || !params) return RT_ERR_INVALID;
     return guarded(ctx, [&] {
-        RT_REQUIRE((flags & ~(RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE)) == 0,
-                   "unknown render flags");
+        RT_REQUIRE((flags & ~kRenderFlags) == 0, "unknown render flags");
         const Scene &s = ctx->scene(scene_id);
         LaunchCfg c = resolve_cfg(ctx, s, cfg, false);
         launch_render(ctx, s, *params, width, height, spp, flags, accum, segments, c,
@@ -185,12 +184,14 @@ int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, 
     return guarded(ctx, [&] {
         if (njobs == 0) return;
         const LaunchCfg c = resolve_cfg(ctx, ctx->scene(jobs[0].scene_id), cfg, false);
+        hipStream_t st = static_cast<hipStream_t>(stream);
         std::map<uint32_t, std::vector<RenderJob>> by_width;
         std::vector<std::pair<uintptr_t, uintptr_t>> ranges;
+        std::vector<RenderJob *> split;
+        uint64_t split_bytes = 0;
         for (uint32_t i = 0; i < njobs; ++i) {
             const rt_render_job &j = jobs[i];
-            RT_REQUIRE((j.flags & ~(RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE)) == 0,
-                       "unknown render flags");
+            RT_REQUIRE((j.flags & ~kRenderFlags) == 0, "unknown render flags");
             const Scene &s = ctx->scene(j.scene_id);
             check_stack(s, c);
             RenderJob rj = make_render_job(s, j.params, j.width, j.height, j.spp, j.flags, j.accum);
@@ -202,7 +203,20 @@ int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, 
         for (size_t i = 1; i < ranges.size(); ++i)
             RT_REQUIRE(ranges[i].first >= ranges[i - 1].second, "rt_render_batch: accumulation images overlap");
         for (auto &kv : by_width)
-            launch_render_jobs(ctx, kv.second, kv.first, segments, c, static_cast<hipStream_t>(stream));
+            for (RenderJob &rj : kv.second)
+                if (prefer_split(ctx, rj.width, rj.height, rj.spp, rj.flags)) {
+                    split.push_back(&rj);
+                    split_bytes += 16ull * rj.width * rj.height * rj.spp;
+                }
+        char *scratch = nullptr;  // one stream-ordered allocation for every split job's samples
+        if (split_bytes) RT_HIP(hipMallocAsync(reinterpret_cast<void **>(&scratch), split_bytes, st));
+        uint64_t off = 0;
+        for (RenderJob *rj : split) {
+            make_split(*rj, reinterpret_cast<float4 *>(scratch + off));
+            off += 16ull * rj->width * rj->height * rj->spp;
+        }
+        for (auto &kv : by_width) launch_render_jobs(ctx, kv.second, kv.first, segments, c, st);
+        if (scratch) RT_HIP(hipFreeAsync(scratch, st));
     });
 }
 

@@ -22,6 +22,14 @@
 // on its accumulator (the alpha carry), rendering spp = a and then spp = b into the
 // same image equals rendering spp = a + b, which lets a scheduler hand out samples
 // in slices.
+//
+// A small image cannot fill the device one pixel per lane, and a lane running a pixel's
+// samples one after another sets the launch's length.  The carry entering sample s is a
+// pure function of the image alpha, the pixel's seed and s (each sample's first draw is
+// the next carry), so a split job gives every (pixel, sample) its own lane: the lane
+// walks the carry chain to its sample (integer and conversion work only), traces that
+// sample and stores its colour; a second pass adds the samples in order, which keeps the
+// image bit-identical.  Split items trace their own first segment.
 #include <algorithm>
 #include <cmath>
 
@@ -145,6 +153,12 @@ __device__ __forceinline__ void begin_sample(const RenderArgs &a, PixelState &ps
 template <bool MULTI>
 __device__ __forceinline__ bool end_sample(const RenderArgs &a, PixelState &ps, Trav &t, uint32_t &segs) {
     const RenderJob &J = job_of<MULTI>(a, ps);
+    if (J.split) {  // one sample per work item: hand it to the in-order sum
+        J.samples[(size_t)ps.sample * (J.width * J.height) + ps.pix] =
+            make_float4(ps.color.x, ps.color.y, ps.color.z, ps.next);
+        ps.phase = kIdle;
+        return false;
+    }
     ps.acc[0] = ps.color.x + ps.acc[0];
     ps.acc[1] = ps.color.y + ps.acc[1];
     ps.acc[2] = ps.color.z + ps.acc[2];
@@ -227,12 +241,36 @@ __device__ __forceinline__ void shade(const RenderArgs &a, PixelState &ps, Trav 
     begin_segment<MULTI>(a, ps, t, segs);
 }
 
-// `local` is the item index within its job
-__device__ __forceinline__ bool item_to_pixel(const RenderJob &J, uint32_t local, uint32_t &x, uint32_t &y) {
-    const uint32_t tile = local >> 6, within = local & 63u;
+// `local` is the item index within its job.  Pixel jobs: 64 items per 8x8 tile.  Split
+// jobs: 64*spp items per tile, sample-major within the tile, so neighbouring lanes trace
+// the same sample of neighbouring pixels.
+__device__ __forceinline__ bool item_to_pixel(const RenderJob &J, uint32_t local, uint32_t &x, uint32_t &y,
+                                              uint32_t &sample) {
+    uint32_t tile, within;
+    if (J.split) {
+        const uint32_t per_tile = 64u * J.spp;
+        tile = local / per_tile;
+        const uint32_t r = local - tile * per_tile;
+        sample = r >> 6;
+        within = r & 63u;
+    } else {
+        tile = local >> 6;
+        within = local & 63u;
+        sample = 0;
+    }
     x = (tile % J.tiles_x) * 8u + (within & 7u);
     y = (tile / J.tiles_x) * 8u + (within >> 3);
     return x < J.width && y < J.height;
+}
+
+// Alpha carry entering sample `s` of a pixel whose image alpha is `a`: each sample seeds
+// its RNG from the carry and its first draw becomes the next carry (raytraceBVH.comp:353-375).
+__device__ __forceinline__ float carry_at(uint32_t seed0, float a, uint32_t s) {
+    for (uint32_t k = 0; k < s; ++k) {
+        uint32_t st = seed0 + f2u_sat(a * 4294967294.0f);
+        a = rng_next(st);
+    }
+    return a;
 }
 
 // last job whose first item is <= g (wave-uniform arguments, so this runs on scalar loads)
@@ -286,8 +324,8 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
             if (ps.phase == kIdle && r < take) {
                 ps.job = q_job;
                 const RenderJob &J = job_of<MULTI>(a, ps);
-                uint32_t x, y;
-                if (item_to_pixel(J, item - J.first_item, x, y)) {
+                uint32_t x, y, sample;
+                if (item_to_pixel(J, item - J.first_item, x, y, sample)) {
                     if constexpr (MULTI) {
                         ps.wide = J.wide;
                         ps.rec = J.rec;
@@ -298,7 +336,8 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
                     const float4 acc = reinterpret_cast<const float4 *>(J.accum)[ps.pix];
                     ps.acc[0] = acc.x; ps.acc[1] = acc.y; ps.acc[2] = acc.z; ps.acc[3] = acc.w;
                     ps.seed0 = rng_seed(x, y, J.random_state);
-                    ps.sample = 0;
+                    ps.sample = sample;
+                    if (J.split) ps.acc[3] = carry_at(ps.seed0, acc.w, sample);  // this item's sample only
                     ps.pc_valid = false;
                     if (J.pcache && J.pcache_valid) {  // traced by an earlier launch over this image
                         const uint2 c = J.pcache[ps.pix];
@@ -332,6 +371,30 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
         if (lane == 0) atomicAdd(a.segments, s);
     }
+}
+
+// In-order sum of a split job's samples into its image: the per-pixel sequence of the
+// reference's spp dispatches (rgb += sample colour, alpha = the sample's carry).  Samples
+// are sample-major ([sample][pixel]) so both the writers and this pass are coalesced.
+__device__ __forceinline__ void sum_job(const RenderJob &J) {
+    if (!J.split) return;
+    const uint32_t npix = J.width * J.height;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+        float4 acc = reinterpret_cast<const float4 *>(J.accum)[p];
+        for (uint32_t k = 0; k < J.spp; ++k) {
+            const float4 v = J.samples[(size_t)k * npix + p];
+            acc.x = v.x + acc.x;
+            acc.y = v.y + acc.y;
+            acc.z = v.z + acc.z;
+            acc.w = v.w;
+        }
+        reinterpret_cast<float4 *>(J.accum)[p] = acc;
+    }
+}
+
+template <bool MULTI> __global__ void k_sum_samples(RenderArgs a) {
+    if constexpr (MULTI) sum_job(a.jobs[blockIdx.y]);
+    else sum_job(a.job0);
 }
 
 __global__ void k_accum_clear(float4 *accum, uint32_t npix) {
@@ -386,7 +449,37 @@ void launch_render_kernel(rt_context *ctx, RenderArgs &a, uint32_t bvh_width, bo
     RT_HIP(hipGetLastError());
 }
 
+// The in-order sum for the split jobs of a launch (max_pix: largest split image).
+void launch_sum_samples(RenderArgs &a, bool multi, uint32_t max_pix, hipStream_t stream) {
+    if (max_pix == 0) return;
+    const uint32_t gx = std::min<uint32_t>((max_pix + 255) / 256, 4096);
+    void *args[] = {&a};
+    void *k = multi ? (void *)k_sum_samples<true> : (void *)k_sum_samples<false>;
+    RT_HIP(hipLaunchKernel(k, dim3(gx, multi ? a.njobs : 1u), dim3(256), args, 0, stream));
+    RT_HIP(hipGetLastError());
+}
+
 }  // namespace
+
+void make_split(RenderJob &j, float4 *samples) {
+    RT_REQUIRE(samples, "split job without sample storage");
+    const uint64_t items = (uint64_t)j.tiles_x * ((j.height + 7) / 8) * 64u * j.spp;
+    RT_REQUIRE(items < (1ull << 32) - kRenderChunk, "too many samples in one split launch");
+    j.split = 1;
+    j.samples = samples;
+    j.nitems = (uint32_t)items;
+    j.pcache = nullptr;  // every item traces its own first segment
+    j.pcache_valid = 0;
+}
+
+bool prefer_split(const rt_context *ctx, uint32_t width, uint32_t height, uint32_t spp, uint32_t flags) {
+    if (flags & RT_RENDER_NO_SPLIT) return false;
+    if (flags & RT_RENDER_SPLIT_SAMPLES) return spp > 0;
+    // about 1024 resident lanes per CU: below ~4 pixels per lane the last pixels' sample chains
+    // leave most of the device idle, which costs more than re-tracing each sample's first segment
+    const uint64_t lanes = 1024ull * (uint64_t)ctx->num_cus;
+    return spp >= 2 && (uint64_t)width * height < 4 * lanes;
+}
 
 void launch_accum_clear(float *accum, uint32_t npix, hipStream_t stream) {
     if (npix == 0) return;
@@ -428,6 +521,8 @@ RenderJob make_render_job(const Scene &sc, const rt_params &p, uint32_t width, u
     j.nitems = j.tiles_x * ((height + 7) / 8) * 64u;
     j.pcache = nullptr;
     j.pcache_valid = 0;
+    j.samples = nullptr;
+    j.split = 0;
     camera_setup(p, width, height, j.cam);
     return j;
 }
@@ -438,6 +533,12 @@ void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_
     RenderJob j = make_render_job(sc, p, width, height, spp, flags, accum);
     if (spp == 0) return;
     check_stack(sc, c);
+    float4 *samples = nullptr;
+    const bool split = prefer_split(ctx, width, height, spp, flags);
+    if (split) {  // stream-ordered scratch from the device's memory pool, freed after the sum
+        RT_HIP(hipMallocAsync(reinterpret_cast<void **>(&samples), 16ull * width * height * spp, stream));
+        make_split(j, samples);
+    }
     RenderArgs a{};
     a.job0 = j;
     a.jobs = nullptr;
@@ -445,18 +546,24 @@ void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_
     a.nitems = j.nitems;
     a.segments = segments;
     launch_render_kernel(ctx, a, sc.width, false, c, stream);
+    if (split) {
+        launch_sum_samples(a, false, width * height, stream);
+        RT_HIP(hipFreeAsync(samples, stream));
+    }
 }
 
 uint64_t launch_render_jobs(rt_context *ctx, std::vector<RenderJob> &jobs, uint32_t bvh_width,
                             unsigned long long *segments, const LaunchCfg &c, hipStream_t stream) {
     uint64_t total = 0;
+    uint32_t max_split_pix = 0;
     for (RenderJob &j : jobs) {
         total = (total + kRenderChunk - 1) / kRenderChunk * kRenderChunk;  // chunks never span jobs
         j.first_item = (uint32_t)total;
         total += j.spp ? j.nitems : 0u;  // a job with no samples gets no items
+        if (j.split && j.spp) max_split_pix = std::max(max_split_pix, j.width * j.height);
     }
     if (total == 0) return 0;
-    RT_REQUIRE(total < (1ull << 32) - kRenderChunk, "too many pixels in one launch");
+    RT_REQUIRE(total < (1ull << 32) - kRenderChunk, "too many work items in one launch");
     int slot = 0;
     void *table = ctx->tables.stage(jobs.data(), sizeof(RenderJob) * jobs.size(), stream, slot);
     RenderArgs a{};
@@ -465,6 +572,7 @@ uint64_t launch_render_jobs(rt_context *ctx, std::vector<RenderJob> &jobs, uint3
     a.nitems = (uint32_t)total;
     a.segments = segments;
     launch_render_kernel(ctx, a, bvh_width, true, c, stream);
+    launch_sum_samples(a, true, max_split_pix, stream);
     ctx->tables.commit(slot, stream);
     return total;
 }

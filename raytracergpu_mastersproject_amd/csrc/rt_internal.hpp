@@ -103,12 +103,20 @@ struct RenderJob {
     const uint32_t *slot_mat;
     const rt_material *mats;
     float *accum;
-    uint2 *pcache;  // optional per-pixel first-segment hit {slot, t bits} kept between launches
+    uint2 *pcache;     // optional per-pixel first-segment hit {slot, t bits} kept between launches
+    float4 *samples;   // split jobs: per (pixel, sample) {colour, next alpha}, pixel-major
     uint32_t nwide, width, height, spp;
     uint32_t depth, random_state, flags, tiles_x;
-    uint32_t first_item, nitems, pcache_valid, _pad0;
+    uint32_t first_item, nitems, pcache_valid, split;
     float cam[12];  // pixel00, deltaU, deltaV, origin
 };
+// Split jobs give every (pixel, sample) its own work item; the samples are summed in
+// order by a second pass.  `samples` must hold width*height*spp entries.
+void make_split(RenderJob &j, float4 *samples);
+// Whether an image is better rendered with one lane per (pixel, sample).
+bool prefer_split(const rt_context *ctx, uint32_t width, uint32_t height, uint32_t spp, uint32_t flags);
+constexpr uint32_t kRenderFlags =
+    RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE | RT_RENDER_SPLIT_SAMPLES | RT_RENDER_NO_SPLIT;
 constexpr uint32_t kRenderChunk = 64;  // work items (one 8x8 tile) per atomic on the render work counter
 
 // Small host->device tables (batch item lists, render job lists) that a launch reads.

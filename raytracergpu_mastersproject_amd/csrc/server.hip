@@ -93,6 +93,7 @@ struct Request {
     uint64_t bytes = 0, offset = 0;
     uint32_t remaining = 0;  // samples (render) or rays (query) not yet given to a step
     uint32_t done_units = 0;
+    bool split = false;  // render with one lane per (pixel, sample)
     rt_request_stats st{};
 };
 
@@ -140,14 +141,19 @@ struct rt_server {
 namespace rtgpu {
 namespace {
 
-// A render holds its accumulation image and, unless primary caching is off, the
-// per-pixel first-segment hits that later slices reuse; a ray query holds rays + hits.
+// Samples per pixel a split render advances per step at most (its sample storage holds this many).
+constexpr uint32_t kSplitSppStep = 16;
+
+// A render holds its accumulation image plus either its per-(pixel, sample) storage (split
+// renders: one lane per sample) or, unless primary caching is off, the per-pixel
+// first-segment hits that later slices reuse.  A ray query holds rays + hits.
 bool keeps_primaries(const rt_request &r) { return !(r.render_flags & RT_RENDER_NO_PRIMARY_CACHE); }
 
-uint64_t request_bytes(const rt_request &r) {
+uint64_t request_bytes(const rt_request &r, bool split) {
     const uint64_t npix = (uint64_t)r.width * r.height;
-    if (r.kind == RT_REQ_RENDER) return Arena::round(16ull * npix) + (keeps_primaries(r) ? Arena::round(8ull * npix) : 0);
-    return Arena::round(32ull * r.nrays) + Arena::round(16ull * r.nrays);
+    if (r.kind != RT_REQ_RENDER) return Arena::round(32ull * r.nrays) + Arena::round(16ull * r.nrays);
+    if (split) return Arena::round(16ull * npix) + Arena::round(16ull * npix * std::min(r.spp, kSplitSppStep));
+    return Arena::round(16ull * npix) + (keeps_primaries(r) ? Arena::round(8ull * npix) : 0);
 }
 
 hipEvent_t take_event(rt_server *s) {
@@ -220,8 +226,10 @@ bool enqueue_step(rt_server *s) {
     bool rc_set = false;
     for (auto &r : renders) {
         const uint64_t npix = (uint64_t)r->req.width * r->req.height;
+        // split renders run one sample per lane, so their slice length does not stretch the step
+        const uint32_t cap = r->split ? kSplitSppStep : s->cfg.max_spp_step;
         uint32_t q = serial ? r->remaining
-                            : (uint32_t)std::min<double>({(double)r->remaining, (double)s->cfg.max_spp_step,
+                            : (uint32_t)std::min<double>({(double)r->remaining, (double)cap,
                                                           std::max(1.0, std::floor(share / (double)npix))});
         q = std::max(1u, std::min(q, r->remaining));
         float *accum = reinterpret_cast<float *>(arena_ptr(s, r->offset));
@@ -237,7 +245,9 @@ bool enqueue_step(rt_server *s) {
                           s->segments.as<unsigned long long>(), rc, s->stream);
         } else {
             RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
-            if (keeps_primaries(r->req)) {
+            if (r->split) {
+                make_split(j, reinterpret_cast<float4 *>(arena_ptr(s, r->offset + Arena::round(16ull * npix))));
+            } else if (keeps_primaries(r->req)) {
                 j.pcache = reinterpret_cast<uint2 *>(arena_ptr(s, r->offset + Arena::round(16ull * npix)));
                 j.pcache_valid = r->done_units > 0 ? 1u : 0u;  // an earlier slice has stored them
             }
@@ -401,7 +411,7 @@ int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **o
         if (cfg) s->cfg = *cfg;
         if (!s->cfg.arena_bytes) s->cfg.arena_bytes = 4ull << 30;
         if (!s->cfg.max_active) s->cfg.max_active = 64;
-        if (!s->cfg.step_us) s->cfg.step_us = 4000;
+        if (!s->cfg.step_us) s->cfg.step_us = 8000;
         if (!s->cfg.pipeline) s->cfg.pipeline = 2;
         s->cfg.pipeline = std::min(s->cfg.pipeline, 8u);
         if (!s->cfg.max_spp_step) s->cfg.max_spp_step = 4;
@@ -457,9 +467,7 @@ int rt_server_submit(rt_server *s, const rt_request *req, uint64_t *id) {
             r->scene = &s->ctx->scene(req->scene_id);
         }
         if (req->kind == RT_REQ_RENDER) {
-            RT_REQUIRE((req->render_flags & ~(RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE |
-                                              RT_REQ_OUT_ACCUM)) == 0,
-                       "unknown render flags");
+            RT_REQUIRE((req->render_flags & ~(kRenderFlags | RT_REQ_OUT_ACCUM)) == 0, "unknown render flags");
             RT_REQUIRE(req->width > 0 && req->height > 0 && (uint64_t)req->width * req->height < (1ull << 31),
                        "bad image size");
             RT_REQUIRE(req->spp > 0, "a render request needs at least one sample per pixel");
@@ -476,7 +484,10 @@ int rt_server_submit(rt_server *s, const rt_request *req, uint64_t *id) {
             check_stack(*r->scene, resolve_cfg(s->ctx, *r->scene, render ? &s->cfg.render_cfg : &s->cfg.trace_cfg,
                                                !render && (req->trace_flags & RT_TRACE_ANY)));
         }
-        r->bytes = request_bytes(*req);
+        if (req->kind == RT_REQ_RENDER)
+            r->split = !(s->cfg.flags & RT_SERVER_SERIAL) && prefer_split(s->ctx, req->width, req->height, req->spp,
+                                                                         req->render_flags);
+        r->bytes = request_bytes(*req, r->split);
     } catch (const Error &e) {
         std::lock_guard<std::mutex> lk(s->m);
         s->last_error = e.what();

@@ -124,6 +124,10 @@ template <int S> __device__ __forceinline__ void pop(Trav &t, const StackMem &m)
     t.have = v.x != kEmpty;
 }
 
+// A ray with a zero direction component that lies exactly in the plane of a box face gets
+// (face - origin) * inf = NaN on that axis and the box is culled, as the reference's
+// AABBhitCheck (raytraceBVH.comp:184-193) culls it: such rays miss primitives that a
+// brute-force test would report.
 __device__ __forceinline__ void trav_init(Trav &t, v3 o, v3 d, float tmin, float tmax, uint32_t nwide) {
     t.o = o;
     t.d = d;

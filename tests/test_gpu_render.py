@@ -1,4 +1,10 @@
-"""Path tracing on the GPU vs the oracle's restatement of raytraceBVH.comp (bit-exact images)."""
+"""Path tracing on the GPU vs the oracle's restatement of raytraceBVH.comp (bit-exact images).
+
+The soup image is compared with the oracle tracing every segment by brute force.  The other
+images are compared with the oracle replaying the GPU's own wide BVH (method WIDE), which is
+brute force except for rays lying exactly in a box-face plane: those are culled like the
+reference traversal culls them (pinned in test_oracle_bvh.py), and an odd image width puts the
+centre column's rays in the plane x = 0 of such scenes."""
 import numpy as np
 import pytest
 
@@ -41,17 +47,40 @@ def test_soup_image_bit_exact(ctx, native, O):
 
 @pytest.mark.parametrize("cache", [True, False])
 def test_spp_chain_and_primary_cache(ctx, native, O, cache):
+    """One lane per pixel (RENDER_NO_SPLIT): the cached first segment saves spp-1 traces per pixel."""
     s = W.soup_scene(width=64, height=48)
     sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
-    flags = 0 if cache else native.RENDER_NO_PRIMARY_CACHE
+    flags = (0 if cache else native.RENDER_NO_PRIMARY_CACHE) | native.RENDER_NO_SPLIT
     g4, segs4 = gpu_render(native, ctx, sid, s["params"], 64, 48, 4, flags)
-    o4, osegs = osc.render(s["params"], 64, 48, 4)
+    o4, osegs = osc.render(s["params"], 64, 48, 4, method=O.WIDE)
     assert np.array_equal(g4.view(np.uint32), o4.view(np.uint32))
     assert segs4 == (osegs - 64 * 48 * 3 if cache else osegs)
     # two successive calls of 2 spp continue the per-pixel chain exactly like one call of 4
     g2, _ = gpu_render(native, ctx, sid, s["params"], 64, 48, 2, flags)
     g22, _ = gpu_render(native, ctx, sid, s["params"], 64, 48, 2, flags, accum=g2)
     assert np.array_equal(g22.view(np.uint32), g4.view(np.uint32))
+    ctx.release(sid)
+
+
+@pytest.mark.parametrize("w,h,spp", [(64, 48, 5), (13, 29, 17), (8, 8, 64)])
+def test_split_samples_equal_the_pixel_chain(ctx, native, O, w, h, spp):
+    """One lane per (pixel, sample), summed in order afterwards, gives the oracle's image bit for bit
+    (including the alpha carry), whether the image starts cleared or holds earlier samples; every
+    sample traces its own first segment."""
+    s = W.icosphere_scene(freq=10, width=w, height=h, spp=spp, depth=6)
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    g, segs = gpu_render(native, ctx, sid, s["params"], w, h, spp, native.RENDER_SPLIT_SAMPLES)
+    o, osegs = osc.render(s["params"], w, h, spp, method=O.WIDE)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    assert segs == osegs
+    # continuing an image: pixel-chain launch then split launch equals the oracle's 2*spp image
+    g1, _ = gpu_render(native, ctx, sid, s["params"], w, h, spp, native.RENDER_NO_SPLIT)
+    g2, _ = gpu_render(native, ctx, sid, s["params"], w, h, spp, native.RENDER_SPLIT_SAMPLES, accum=g1)
+    o2, _ = osc.render(s["params"], w, h, 2 * spp, method=O.WIDE)
+    assert np.array_equal(g2.view(np.uint32), o2.view(np.uint32))
+    # the default picks the split layout for an image this small, with the same image
+    gd, _ = gpu_render(native, ctx, sid, s["params"], w, h, spp)
+    assert np.array_equal(gd.view(np.uint32), o.view(np.uint32))
     ctx.release(sid)
 
 
@@ -66,7 +95,7 @@ def test_metal_dielectric_scene(ctx, native, O, ref_mats):
     sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
     flags = native.RENDER_REFERENCE_MATERIALS if ref_mats else 0
     g, _ = gpu_render(native, ctx, sid, s["params"], 96, 64, 2, flags)
-    o, _ = osc.render(s["params"], 96, 64, 2, flags=O.RENDER_REFERENCE_MATERIALS if ref_mats else 0)
+    o, _ = osc.render(s["params"], 96, 64, 2, flags=O.RENDER_REFERENCE_MATERIALS if ref_mats else 0, method=O.WIDE)
     assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
     ctx.release(sid)
 
@@ -76,7 +105,7 @@ def test_spheres_scene(ctx, native, O):
     sph = with_spheres(s["tris"], s["materials"], 6)
     sid, osc = upload_both(ctx, O, s["tris"], sph, s["materials"])
     g, _ = gpu_render(native, ctx, sid, s["params"], 80, 60, 2)
-    o, _ = osc.render(s["params"], 80, 60, 2)
+    o, _ = osc.render(s["params"], 80, 60, 2, method=O.WIDE)
     assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
     ctx.release(sid)
 
@@ -87,7 +116,7 @@ def test_rows_beyond_600_and_odd_sizes(ctx, native, O):
     sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
     w, h = 13, 611
     g, _ = gpu_render(native, ctx, sid, s["params"], w, h, 1)
-    o, _ = osc.render(s["params"], w, h, 1)
+    o, _ = osc.render(s["params"], w, h, 1, method=O.WIDE)
     assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
     ctx.release(sid)
 
@@ -109,7 +138,7 @@ def test_depth_zero_and_empty_scene(ctx, native, O):
     p["maxRayTraceDepth"] = 0
     sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
     g, segs = gpu_render(native, ctx, sid, p, 16, 16, 3)
-    o, _ = osc.render(p, 16, 16, 3)
+    o, _ = osc.render(p, 16, 16, 3, method=O.WIDE)
     assert segs == 0 and np.array_equal(g.view(np.uint32), o.view(np.uint32))
     ctx.release(sid)
     sid = ctx.upload(None, None, s["materials"])

@@ -102,9 +102,10 @@ def test_render_batch_rejects_overlapping_images(ctx, native, scenes):
 def _mixed_requests(native, ids, params, rays_by_scene):
     """Render and ray-query requests over several scenes, sizes and sample counts."""
     reqs, expect = [], []
-    spec = [("soup4", 64, 48, 6, 0, 1), ("ico", 48, 40, 3, 0, 2), ("soup8", 80, 64, 2, 0, 3),
-            ("ico2", 24, 24, 12, 1, 4), ("soup4", 17, 33, 7, 2, 5), ("ico", 96, 64, 1, 0, 6),
-            ("soup4", 128, 96, 3, 0, 7), ("ico2", 32, 32, 5, 0, 8)]
+    # flags 8 (RENDER_NO_SPLIT) keeps one lane per pixel, so both sample layouts are served
+    spec = [("soup4", 64, 48, 6, 0, 1), ("ico", 48, 40, 11, 8, 2), ("soup8", 80, 64, 2, 0, 3),
+            ("ico2", 24, 24, 40, 1, 4), ("soup4", 17, 33, 7, 2, 5), ("ico", 96, 64, 1, 0, 6),
+            ("soup4", 128, 96, 9, 8, 7), ("ico2", 32, 32, 5, 10, 8)]
     for name, w, h, spp, flags, rs in spec:
         p = with_state(params[name], rs)
         out = np.zeros((w * h, 4), np.float32)

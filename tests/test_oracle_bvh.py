@@ -211,3 +211,35 @@ def test_ties_choose_the_primitive_tested_last():
     hit = hb["prim"] != O.NO_HIT
     twin = np.where(hb["prim"][hit] < 300, hb["prim"][hit] + 300, hb["prim"][hit] - 300)
     assert np.all(slot_of[hb["prim"][hit]] < slot_of[twin])
+
+
+def test_rays_in_a_box_face_plane_are_culled_like_the_reference_traversal():
+    """A ray with a zero direction component lying exactly in the plane of a box face gives
+    (face - origin) * inf = NaN on that axis.  The reference's AABBhitCheck (raytraceBVH.comp:184-193,
+    restated with GLSL min/max) culls the box, and so does the product's traversal (replayed here
+    over a hand-built two-leaf wide node, IEEE minNum/maxNum slabs); brute force reports the hit."""
+    t = np.zeros(2, O.TRIANGLE)
+    t["v0"][0, :3], t["v1"][0, :3], t["v2"][0, :3] = [0, -1, 1], [1, -1, 1], [0, 1, 1]      # x in [0, 1], z = 1
+    t["v0"][1, :3], t["v1"][1, :3], t["v2"][1, :3] = [2, -1, 4], [3, -1, 4], [3, 1, 4]      # x in [2, 3], z = 4
+    t["v0"][:, 3] = t["v1"][:, 3] = t["v2"][:, 3] = 1.0
+    mats = W.make_materials([0.5, 0.5, 0.5], 1)
+    width = 2
+    node = np.zeros(8 * width, np.uint32)
+    f = node.view(np.float32)
+    for c in range(2):
+        v = np.stack([t["v0"][c, :3], t["v1"][c, :3], t["v2"][c, :3]])
+        lo, hi = v.min(0), v.max(0)
+        for k in range(3):
+            f[(2 * k) * width + c], f[(2 * k + 1) * width + c] = lo[k], hi[k]
+        node[6 * width + c] = (1 << 31) | c  # leaf: one primitive at slot c
+    wide_scene = O.OracleScene(t, None, mats, None, np.array([0, 1], np.uint32), node, width)
+    ref_scene = O.OracleScene(t, None, mats)  # the reference LBVH over the same triangles
+    r = np.zeros(4, O.RAY)
+    r["tmin"], r["tmax"] = 0.001, 1e7
+    r["o"] = [[0, 0, 0], [0, 0, 0], [3, 0.5, 0], [0.5, 0, 0]]
+    r["d"] = [[0, 0, 1], [-0.0, 0, 1], [0, 0, 1], [0, 0, 1]]  # faces x = 0 (both zero signs), x = 3; inside
+    brute = ref_scene.trace(r, O.BRUTE)
+    assert list(brute["prim"]) == [0, 0, 1, 0]
+    for hits in (ref_scene.trace(r, O.REF_BVH), wide_scene.trace(r, O.WIDE)):
+        assert list(hits["prim"][:3]) == [O.NO_HIT] * 3  # culled in the face plane
+        assert hits["prim"][3] == 0 and hits["t"][3] == brute["t"][3]  # strictly inside: found
