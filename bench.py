@@ -122,15 +122,15 @@ def pathtrace_bench(ctx, N, W, L, frames: int, cpu_seconds: float, threads: int)
     total_ms = ev[0].elapsed_ms(ev[-1])
     ms = total_ms / frames
     seg = int(segs.to_numpy(np.uint64, 1)[0]) // frames
-    seg_all = seg + npix * (spp - 1)  # primary segments reused for samples 2..spp
+    # traced segments depend on how often a first segment is re-traced (once per pixel, or once
+    # per group of samples in the split layout); pixel-samples per second compare across layouts
     out = {"workload": f"path trace: {len(s['tris'])} triangles (icosphere freq 224, noise-displaced, per-triangle "
                        f"lambertian/metal/dielectric) + emissive quad, {w}x{h}, {spp} spp, depth 8",
            "ms_per_frame": round(ms, 3), "segments_traced_per_frame": seg,
-           "segments_per_frame_incl_reused_primaries": seg_all,
            "msegments_per_s": round(seg / (ms * 1e-3) / 1e6, 1),
            "msamples_per_s": round(npix * spp / (ms * 1e-3) / 1e6, 1),
            "launch": dict(zip(("block", "lds_stack", "persistent", "refill"), best[0]))}
-    log(f"pathtrace: {ms:.2f} ms/frame, {out['msegments_per_s']} Msegments/s")
+    log(f"pathtrace: {ms:.2f} ms/frame, {out['msamples_per_s']} Msamples/s, {out['msegments_per_s']} Msegments/s")
     if cpu_seconds > 0:
         import oracle as O
         nodes, _, slots, _, _ = O.build_lbvh(s["tris"])
@@ -313,8 +313,8 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serve-requests", type=int, default=160)
-    ap.add_argument("--serve-arena-gib", type=float, default=2.0)
-    ap.add_argument("--serve-step-us", type=int, default=8000)
+    ap.add_argument("--serve-arena-gib", type=float, default=16.0)
+    ap.add_argument("--serve-step-us", type=int, default=50000)
     ap.add_argument("--serve-max-spp-step", type=int, default=0)
     args = ap.parse_args()
     if args.copy_calibration:

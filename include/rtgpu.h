@@ -292,10 +292,11 @@ typedef struct rt_server rt_server;
 typedef struct rt_server_config {
     uint64_t arena_bytes;        /* device memory for request buffers: the admission budget (0 = 4 GiB) */
     uint32_t max_active;         /* requests admitted at once (0 = 64) */
-    uint32_t step_us;            /* target GPU time of one scheduling step (0 = 8000) */
+    uint32_t step_us;            /* GPU time a scheduling step may take at most (0 = 50000) */
     uint32_t pipeline;           /* steps enqueued ahead of the device (0 = 2) */
     uint32_t flags;              /* RT_SERVER_* */
-    uint32_t max_spp_step;       /* samples per pixel a render advances per step at most (0 = 4) */
+    uint32_t max_spp_step;       /* samples per pixel a render advances per step at most (0 = no limit beyond
+                                    the step budget); a step holds 16 B per pixel-sample of its renders */
     uint32_t _reserved;
     rt_launch_config render_cfg; /* zeros = defaults */
     rt_launch_config trace_cfg;

@@ -187,7 +187,7 @@ int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, 
         hipStream_t st = static_cast<hipStream_t>(stream);
         std::map<uint32_t, std::vector<RenderJob>> by_width;
         std::vector<std::pair<uintptr_t, uintptr_t>> ranges;
-        std::vector<RenderJob *> split;
+        std::vector<std::pair<RenderJob *, uint32_t>> split;  // (job, samples per item)
         uint64_t split_bytes = 0;
         for (uint32_t i = 0; i < njobs; ++i) {
             const rt_render_job &j = jobs[i];
@@ -204,15 +204,15 @@ int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, 
             RT_REQUIRE(ranges[i].first >= ranges[i - 1].second, "rt_render_batch: accumulation images overlap");
         for (auto &kv : by_width)
             for (RenderJob &rj : kv.second)
-                if (prefer_split(ctx, rj.width, rj.height, rj.spp, rj.flags)) {
-                    split.push_back(&rj);
+                if (const uint32_t g = split_group(ctx, rj.width, rj.height, rj.spp, rj.flags)) {
+                    split.emplace_back(&rj, g);
                     split_bytes += 16ull * rj.width * rj.height * rj.spp;
                 }
         char *scratch = nullptr;  // one stream-ordered allocation for every split job's samples
         if (split_bytes) RT_HIP(hipMallocAsync(reinterpret_cast<void **>(&scratch), split_bytes, st));
         uint64_t off = 0;
-        for (RenderJob *rj : split) {
-            make_split(*rj, reinterpret_cast<float4 *>(scratch + off));
+        for (auto &[rj, g] : split) {
+            make_split(*rj, reinterpret_cast<float4 *>(scratch + off), g);
             off += 16ull * rj->width * rj->height * rj->spp;
         }
         for (auto &kv : by_width) launch_render_jobs(ctx, kv.second, kv.first, segments, c, st);

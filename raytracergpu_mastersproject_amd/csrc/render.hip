@@ -32,6 +32,7 @@
 // image bit-identical.  Split items trace their own first segment.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "rt_internal.hpp"
 #include "traverse.hpp"
@@ -70,7 +71,7 @@ namespace {
 struct RenderArgs {
     RenderJob job0;         // single-image launches
     const RenderJob *jobs;  // multi-image launches (device table)
-    uint32_t njobs, nitems, refill, leaf_weight;
+    uint32_t njobs, nitems, refill, leaf_weight, shade_min;
     unsigned long long *segments;
     unsigned int *work;
 };
@@ -153,9 +154,15 @@ __device__ __forceinline__ void begin_sample(const RenderArgs &a, PixelState &ps
 template <bool MULTI>
 __device__ __forceinline__ bool end_sample(const RenderArgs &a, PixelState &ps, Trav &t, uint32_t &segs) {
     const RenderJob &J = job_of<MULTI>(a, ps);
-    if (J.split) {  // one sample per work item: hand it to the in-order sum
+    if (J.group) {  // split job: hand the sample to the in-order sum, then the item's next one
         J.samples[(size_t)ps.sample * (J.width * J.height) + ps.pix] =
             make_float4(ps.color.x, ps.color.y, ps.color.z, ps.next);
+        ps.sample++;
+        if (ps.sample < J.spp && ps.sample % J.group != 0) {
+            ps.acc[3] = ps.next;
+            begin_sample<MULTI>(a, ps, t, segs);
+            return true;
+        }
         ps.phase = kIdle;
         return false;
     }
@@ -181,7 +188,6 @@ __device__ __forceinline__ void shade(const RenderArgs &a, PixelState &ps, Trav 
         ps.pc_slot = ps.hslot;
         ps.pc_t = ps.ht;
         ps.pc_valid = true;
-        if (J.pcache) J.pcache[ps.pix] = make_uint2(ps.hslot, __float_as_uint(ps.ht));  // for later launches
     }
     if (ps.hslot == kEmpty) {
         const v3 background{0.0f, 0.0f, 0.0f};  // _BACKGROUND_COLOR
@@ -242,16 +248,17 @@ __device__ __forceinline__ void shade(const RenderArgs &a, PixelState &ps, Trav 
 }
 
 // `local` is the item index within its job.  Pixel jobs: 64 items per 8x8 tile.  Split
-// jobs: 64*spp items per tile, sample-major within the tile, so neighbouring lanes trace
-// the same sample of neighbouring pixels.
+// jobs: 64 items per group of `group` samples per tile, group-major within the tile, so
+// neighbouring lanes trace the same samples of neighbouring pixels; `sample` is the
+// item's first sample.
 __device__ __forceinline__ bool item_to_pixel(const RenderJob &J, uint32_t local, uint32_t &x, uint32_t &y,
                                               uint32_t &sample) {
     uint32_t tile, within;
-    if (J.split) {
-        const uint32_t per_tile = 64u * J.spp;
+    if (J.group) {
+        const uint32_t per_tile = 64u * ((J.spp + J.group - 1) / J.group);
         tile = local / per_tile;
         const uint32_t r = local - tile * per_tile;
-        sample = r >> 6;
+        sample = (r >> 6) * J.group;
         within = r & 63u;
     } else {
         tile = local >> 6;
@@ -337,34 +344,36 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
                     ps.acc[0] = acc.x; ps.acc[1] = acc.y; ps.acc[2] = acc.z; ps.acc[3] = acc.w;
                     ps.seed0 = rng_seed(x, y, J.random_state);
                     ps.sample = sample;
-                    if (J.split) ps.acc[3] = carry_at(ps.seed0, acc.w, sample);  // this item's sample only
+                    if (J.group) ps.acc[3] = carry_at(ps.seed0, acc.w, sample);  // this item's samples only
                     ps.pc_valid = false;
-                    if (J.pcache && J.pcache_valid) {  // traced by an earlier launch over this image
-                        const uint2 c = J.pcache[ps.pix];
-                        ps.pc_slot = c.x;
-                        ps.pc_t = __uint_as_float(c.y);
-                        ps.pc_valid = true;
-                    }
                     if (J.spp == 0) ps.phase = kIdle;
                     else begin_sample<MULTI>(a, ps, t, segs);
                 }
             }
         }
         if (__ballot(ps.phase != kIdle) == 0ull) break;
-        const bool tracing = ps.phase == kTracing;
-        const bool on_node = tracing && at_node(t), on_leaf = tracing && at_leaf(t);
-        if (choose_leaf_phase(on_node, on_leaf, a.leaf_weight)) {
-            if (on_leaf) leaf_step<S, false>(t, m, rec_of<MULTI>(a, ps));
+        // Shading (RNG draws, sampling, scatter, the next segment's setup) is long and only a
+        // few lanes finish a segment per iteration, so a wave shades only when at least
+        // shade_min of its lanes hold a resolved segment, or when none is left tracing; the
+        // resolved lanes wait meanwhile.  A lane's own sequence of segments is unchanged.
+        const unsigned long long mres = __ballot(ps.phase == kResolved), mtr = __ballot(ps.phase == kTracing);
+        if (mres != 0ull && (mtr == 0ull || (uint32_t)__popcll(mres) >= a.shade_min)) {
+            // resolved segments may chain (cached primary hit, depth reached): bounded by spp*depth
+            while (ps.phase == kResolved) shade<MULTI>(a, ps, t, segs);
         } else {
-            if (on_node) node_step<W, S, false>(t, m, wide_of<MULTI>(a, ps));
+            const bool tracing = ps.phase == kTracing;
+            const bool on_node = tracing && at_node(t), on_leaf = tracing && at_leaf(t);
+            if (choose_leaf_phase(on_node, on_leaf, a.leaf_weight)) {
+                if (on_leaf) leaf_step<S, false>(t, m, rec_of<MULTI>(a, ps));
+            } else {
+                if (on_node) node_step<W, S, false>(t, m, wide_of<MULTI>(a, ps));
+            }
+            if (tracing && trav_done(t)) {
+                ps.hslot = t.bslot;
+                ps.ht = t.bt;
+                ps.phase = kResolved;
+            }
         }
-        if (tracing && trav_done(t)) {
-            ps.hslot = t.bslot;
-            ps.ht = t.bt;
-            ps.phase = kResolved;
-        }
-        // resolved segments may chain (cached primary hit, depth reached): bounded by spp*depth
-        while (ps.phase == kResolved) shade<MULTI>(a, ps, t, segs);
     }
     if (a.segments) {
         unsigned long long s = segs;
@@ -377,7 +386,7 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
 // reference's spp dispatches (rgb += sample colour, alpha = the sample's carry).  Samples
 // are sample-major ([sample][pixel]) so both the writers and this pass are coalesced.
 __device__ __forceinline__ void sum_job(const RenderJob &J) {
-    if (!J.split) return;
+    if (!J.group) return;
     const uint32_t npix = J.width * J.height;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
         float4 acc = reinterpret_cast<const float4 *>(J.accum)[p];
@@ -434,6 +443,7 @@ void launch_render_kernel(rt_context *ctx, RenderArgs &a, uint32_t bvh_width, bo
                           hipStream_t stream) {
     a.refill = c.refill;
     a.leaf_weight = c.leaf_weight;
+    a.shade_min = c.shade_min;
     a.work = ctx->counter_slot();
     k_zero<<<1, 64, 0, stream>>>(a.work);
     const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
@@ -461,24 +471,34 @@ void launch_sum_samples(RenderArgs &a, bool multi, uint32_t max_pix, hipStream_t
 
 }  // namespace
 
-void make_split(RenderJob &j, float4 *samples) {
+void make_split(RenderJob &j, float4 *samples, uint32_t group) {
     RT_REQUIRE(samples, "split job without sample storage");
-    const uint64_t items = (uint64_t)j.tiles_x * ((j.height + 7) / 8) * 64u * j.spp;
+    RT_REQUIRE(group >= 1, "split group must hold at least one sample");
+    const uint64_t groups = (j.spp + group - 1) / group;
+    const uint64_t items = (uint64_t)j.tiles_x * ((j.height + 7) / 8) * 64u * groups;
     RT_REQUIRE(items < (1ull << 32) - kRenderChunk, "too many samples in one split launch");
-    j.split = 1;
+    j.group = group;
     j.samples = samples;
-    j.nitems = (uint32_t)items;
-    j.pcache = nullptr;  // every item traces its own first segment
-    j.pcache_valid = 0;
+    j.nitems = (uint32_t)items;  // each item traces its first segment once for its own samples
 }
 
-bool prefer_split(const rt_context *ctx, uint32_t width, uint32_t height, uint32_t spp, uint32_t flags) {
-    if (flags & RT_RENDER_NO_SPLIT) return false;
-    if (flags & RT_RENDER_SPLIT_SAMPLES) return spp > 0;
-    // about 1024 resident lanes per CU: below ~4 pixels per lane the last pixels' sample chains
-    // leave most of the device idle, which costs more than re-tracing each sample's first segment
-    const uint64_t lanes = 1024ull * (uint64_t)ctx->num_cus;
-    return spp >= 2 && (uint64_t)width * height < 4 * lanes;
+namespace {
+uint64_t device_lanes(const rt_context *ctx) { return 1024ull * (uint64_t)ctx->num_cus; }  // ~resident lanes
+}  // namespace
+
+uint32_t split_group(const rt_context *ctx, uint32_t width, uint32_t height, uint32_t spp, uint32_t flags) {
+    if (flags & RT_RENDER_NO_SPLIT || spp < 2) return 0;
+    if (flags & RT_RENDER_SPLIT_SAMPLES) return 1;
+    // Work items of `group` samples, enough of them (~32 per resident lane) that the launch's
+    // drain -- lanes finishing their last, longest items -- is short; each item traces its
+    // first segment once.  Whole pixels when that already holds, or when the per-sample
+    // storage would be unreasonably large.
+    uint64_t per_lane = 32;
+    if (const char *e = std::getenv("RTGPU_ITEMS_PER_LANE")) per_lane = (uint64_t)std::max(1, std::atoi(e));
+    const uint64_t samples = (uint64_t)width * height * spp;
+    const uint64_t group = std::max<uint64_t>(1, samples / (per_lane * device_lanes(ctx)));
+    if (group >= spp || 16ull * samples > (4ull << 30)) return 0;
+    return (uint32_t)group;
 }
 
 void launch_accum_clear(float *accum, uint32_t npix, hipStream_t stream) {
@@ -519,10 +539,8 @@ RenderJob make_render_job(const Scene &sc, const rt_params &p, uint32_t width, u
     j.tiles_x = (width + 7) / 8;
     j.first_item = 0;
     j.nitems = j.tiles_x * ((height + 7) / 8) * 64u;
-    j.pcache = nullptr;
-    j.pcache_valid = 0;
     j.samples = nullptr;
-    j.split = 0;
+    j.group = 0;
     camera_setup(p, width, height, j.cam);
     return j;
 }
@@ -534,10 +552,11 @@ void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_
     if (spp == 0) return;
     check_stack(sc, c);
     float4 *samples = nullptr;
-    const bool split = prefer_split(ctx, width, height, spp, flags);
+    const uint32_t group = split_group(ctx, width, height, spp, flags);
+    const bool split = group != 0;
     if (split) {  // stream-ordered scratch from the device's memory pool, freed after the sum
         RT_HIP(hipMallocAsync(reinterpret_cast<void **>(&samples), 16ull * width * height * spp, stream));
-        make_split(j, samples);
+        make_split(j, samples, group);
     }
     RenderArgs a{};
     a.job0 = j;
@@ -560,7 +579,7 @@ uint64_t launch_render_jobs(rt_context *ctx, std::vector<RenderJob> &jobs, uint3
         total = (total + kRenderChunk - 1) / kRenderChunk * kRenderChunk;  // chunks never span jobs
         j.first_item = (uint32_t)total;
         total += j.spp ? j.nitems : 0u;  // a job with no samples gets no items
-        if (j.split && j.spp) max_split_pix = std::max(max_split_pix, j.width * j.height);
+        if (j.group && j.spp) max_split_pix = std::max(max_split_pix, j.width * j.height);
     }
     if (total == 0) return 0;
     RT_REQUIRE(total < (1ull << 32) - kRenderChunk, "too many work items in one launch");

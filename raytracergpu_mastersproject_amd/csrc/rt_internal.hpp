@@ -71,6 +71,7 @@ struct LaunchCfg {
     uint32_t persistent = 1;
     uint32_t refill = 16;
     uint32_t leaf_weight = 2;  // leaf phase when leaf lanes * weight >= node lanes
+    uint32_t shade_min = 24;   // render: shade when this many lanes hold a resolved segment
 };
 
 // A scene resident in HBM.
@@ -103,56 +104,65 @@ struct RenderJob {
     const uint32_t *slot_mat;
     const rt_material *mats;
     float *accum;
-    uint2 *pcache;     // optional per-pixel first-segment hit {slot, t bits} kept between launches
-    float4 *samples;   // split jobs: per (pixel, sample) {colour, next alpha}, pixel-major
+    float4 *samples;   // split jobs: per (pixel, sample) {colour, next carry}, [sample][pixel]
     uint32_t nwide, width, height, spp;
     uint32_t depth, random_state, flags, tiles_x;
-    uint32_t first_item, nitems, pcache_valid, split;
+    uint32_t first_item, nitems, group, _pad;  // group: samples per split item (0 = whole pixels)
     float cam[12];  // pixel00, deltaU, deltaV, origin
 };
-// Split jobs give every (pixel, sample) its own work item; the samples are summed in
-// order by a second pass.  `samples` must hold width*height*spp entries.
-void make_split(RenderJob &j, float4 *samples);
-// Whether an image is better rendered with one lane per (pixel, sample).
-bool prefer_split(const rt_context *ctx, uint32_t width, uint32_t height, uint32_t spp, uint32_t flags);
+// Split jobs give every (pixel, group of `group` consecutive samples) its own work item; the
+// samples are summed in order by a second pass.  `samples` must hold width*height*spp entries.
+void make_split(RenderJob &j, float4 *samples, uint32_t group);
+// Samples per split item for a whole-image render (0 = one lane per pixel).
+uint32_t split_group(const rt_context *ctx, uint32_t width, uint32_t height, uint32_t spp, uint32_t flags);
 constexpr uint32_t kRenderFlags =
     RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE | RT_RENDER_SPLIT_SAMPLES | RT_RENDER_NO_SPLIT;
 constexpr uint32_t kRenderChunk = 64;  // work items (one 8x8 tile) per atomic on the render work counter
 
 // Small host->device tables (batch item lists, render job lists) that a launch reads.
 // Each slot has a pinned host buffer and a device buffer; a slot is reused only after
-// the launch that read it has finished (its event), so launches need no host sync.
+// the launch that read it has finished, so launches need no host sync.  Completion is
+// tracked on the ring's own stream: commit() marks the caller's stream, makes the ring
+// stream wait for that mark and records the slot's `done` there.  The host only ever
+// waits on events recorded on the ring stream, which lives as long as the ring, never on
+// events of a caller's stream that may have been destroyed since.
 class TableRing {
     static constexpr int kSlots = 32;
     struct Slot {
         void *host = nullptr;
         DevBuf dev;
         size_t cap = 0;
-        hipEvent_t done = nullptr;
+        hipEvent_t mark = nullptr;  // recorded on the caller's stream, waited on by the ring stream only
+        hipEvent_t done = nullptr;  // recorded on the ring stream
         bool pending = false;
     };
     Slot slots_[kSlots];
     uint32_t next_ = 0;
+    hipStream_t ring_ = nullptr;
 
 public:
     TableRing() = default;
     TableRing(const TableRing &) = delete;
     TableRing &operator=(const TableRing &) = delete;
     ~TableRing() {
+        if (ring_) (void)hipStreamSynchronize(ring_);
         for (Slot &s : slots_) {
-            if (s.done) {
-                (void)hipEventSynchronize(s.done);
-                (void)hipEventDestroy(s.done);
-            }
+            if (s.mark) (void)hipEventDestroy(s.mark);
+            if (s.done) (void)hipEventDestroy(s.done);
             if (s.host) (void)hipHostFree(s.host);
         }
+        if (ring_) (void)hipStreamDestroy(ring_);
     }
     // Enqueue a copy of `bytes` from `src` into a device table on `stream`.  Call
     // commit(slot, stream) after the launch that reads the table.
     void *stage(const void *src, size_t bytes, hipStream_t stream, int &slot) {
+        if (!ring_) RT_HIP(hipStreamCreateWithFlags(&ring_, hipStreamNonBlocking));
         slot = (int)(next_++ % kSlots);
         Slot &s = slots_[slot];
-        if (!s.done) RT_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        if (!s.done) {
+            RT_HIP(hipEventCreateWithFlags(&s.mark, hipEventDisableTiming));
+            RT_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        }
         if (s.pending) RT_HIP(hipEventSynchronize(s.done));
         s.pending = false;
         if (s.cap < bytes) {
@@ -167,13 +177,14 @@ public:
         }
         std::memcpy(s.host, src, bytes);
         RT_HIP(hipMemcpyAsync(s.dev.p, s.host, bytes, hipMemcpyHostToDevice, stream));
-        RT_HIP(hipEventRecord(s.done, stream));
-        s.pending = true;
+        commit(slot, stream);  // covers the copy even if the launch is never enqueued
         return s.dev.p;
     }
     void commit(int slot, hipStream_t stream) {
         Slot &s = slots_[slot];
-        RT_HIP(hipEventRecord(s.done, stream));
+        RT_HIP(hipEventRecord(s.mark, stream));
+        RT_HIP(hipStreamWaitEvent(ring_, s.mark, 0));
+        RT_HIP(hipEventRecord(s.done, ring_));
         s.pending = true;
     }
 };

@@ -1,15 +1,17 @@
 // server.hip -- the serving loop: resident scenes, budgeted admission, continuous batching.
 //
 // Requests wait in a FIFO queue.  The head is admitted when a slot is free and the
-// request's buffers fit in the server's device arena (first-fit allocation in one
-// hipMalloc made at creation, so admission never calls the allocator).  Every admitted
-// request is advanced in scheduling steps: a step is one fused render launch over a
-// slice of samples of every admitted render (launch_render_jobs) plus one fused
-// ray-query launch over a slice of rays of every admitted query (launch_trace_items),
-// enqueued on the server's stream up to `pipeline` steps ahead of the device.  A
-// request that joins while others run takes its share from the next step on, and the
-// slices are sized so a step lasts about `step_us` of device time (rates measured
-// from earlier steps).  A request's last slice is followed by its resolve; when that
+// request's buffers fit in the first half of the server's device arena (first-fit
+// allocation in one hipMalloc made at creation, so admission never calls the allocator).
+// Admitted requests advance in scheduling steps, enqueued on the server's stream up to
+// `pipeline` steps ahead of the device.  A step is one fused render launch
+// (launch_render_jobs) plus one fused ray-query launch over slices of every admitted
+// query (launch_trace_items).  The render launch carries either the oldest render alone,
+// when it fills the device by itself, or the admitted small renders together, whole
+// where they fit the step's budget of device time (rates measured from earlier steps);
+// renders larger than the budget are cut into slices of samples.  Renders run in the
+// split layout, their per-sample storage taken from the second half of the arena for
+// the step's lifetime.  A request's last slice is followed by its resolve; when that
 // step's event completes the result is copied out and its arena range freed.
 //
 // A render's samples may be cut into any number of slices without changing its image
@@ -93,7 +95,6 @@ struct Request {
     uint64_t bytes = 0, offset = 0;
     uint32_t remaining = 0;  // samples (render) or rays (query) not yet given to a step
     uint32_t done_units = 0;
-    bool split = false;  // render with one lane per (pixel, sample)
     rt_request_stats st{};
 };
 
@@ -102,6 +103,7 @@ struct Step {
     std::vector<std::shared_ptr<Request>> finishing;
     std::vector<std::shared_ptr<Request>> members;
     uint64_t samples = 0, rays = 0;
+    uint64_t scratch_off = 0, scratch_bytes = 0;  // per-sample storage of the step's renders
 };
 
 }  // namespace
@@ -111,7 +113,9 @@ struct rt_server {
     rt_context *ctx = nullptr;
     rt_server_config cfg{};
     rtgpu::DevBuf arena_mem;
-    rtgpu::Arena arena;
+    rtgpu::Arena arena;    // first half of arena_mem: admitted requests' images, rays and hits
+    rtgpu::Arena scratch;  // second half: per-sample storage of the steps in flight
+    uint64_t scratch_base = 0;
     rtgpu::DevBuf segments;  // u64 device counter of traced render segments
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> event_pool;
@@ -141,20 +145,16 @@ struct rt_server {
 namespace rtgpu {
 namespace {
 
-// Samples per pixel a split render advances per step at most (its sample storage holds this many).
-constexpr uint32_t kSplitSppStep = 16;
-
-// A render holds its accumulation image plus either its per-(pixel, sample) storage (split
-// renders: one lane per sample) or, unless primary caching is off, the per-pixel
-// first-segment hits that later slices reuse.  A ray query holds rays + hits.
-bool keeps_primaries(const rt_request &r) { return !(r.render_flags & RT_RENDER_NO_PRIMARY_CACHE); }
-
-uint64_t request_bytes(const rt_request &r, bool split) {
-    const uint64_t npix = (uint64_t)r.width * r.height;
+// A render holds its accumulation image plus, when its slices run in the split layout, the
+// per-(pixel, sample) storage of one slice (at most max_spp_step samples).  The unbatched
+// server renders whole requests through rt_render's own scratch.  A ray query holds rays + hits.
+uint64_t request_bytes(const rt_request &r) {
     if (r.kind != RT_REQ_RENDER) return Arena::round(32ull * r.nrays) + Arena::round(16ull * r.nrays);
-    if (split) return Arena::round(16ull * npix) + Arena::round(16ull * npix * std::min(r.spp, kSplitSppStep));
-    return Arena::round(16ull * npix) + (keeps_primaries(r) ? Arena::round(8ull * npix) : 0);
+    return Arena::round(16ull * r.width * r.height);
 }
+
+// resident lanes of the device, about 1024 per CU
+uint64_t device_lanes(const rt_server *s) { return 1024ull * (uint64_t)s->ctx->num_cus; }
 
 hipEvent_t take_event(rt_server *s) {
     if (!s->event_pool.empty()) {
@@ -210,28 +210,80 @@ bool enqueue_step(rt_server *s) {
         queries.clear();
         (first->req.kind == RT_REQ_RENDER ? renders : queries).push_back(first);
     }
+    // Renders this step, in admission order: whole while they fit the step's sample budget;
+    // the first that does not fit gets the number of passes that does (at least one when it
+    // is the step's first render).  Every job runs in the split layout with one group size
+    // for the launch, chosen so the launch has ~32 work items per resident lane: its drain
+    // (lanes finishing their last items) stays short and each item traces its first segment
+    // once for its group of samples.  The per-sample storage comes from the scratch half of
+    // the arena and returns there when the step retires.
+    //
+    // A render that fills the device by itself (at least ~32 samples per resident lane) runs
+    // alone, as the unbatched path would run it, so the small requests behind it are not
+    // held until the end of a launch that also carries it; small renders are fused.  The
+    // oldest unfinished render decides which kind of step this is.
+    // A step of fused small renders finishes all of them together, so it gets a fifth of the
+    // budget: its first members are not held long by its last.
+    double budget = std::max(1.0, s->sample_rate * s->cfg.step_us);
+    std::vector<std::pair<std::shared_ptr<Request>, uint32_t>> plan;
+    uint64_t total = 0;
+    auto npix_of = [](const Request &r) { return (uint64_t)r.req.width * r.req.height; };
+    const uint64_t fills = 32ull * device_lanes(s);
+    auto is_big = [&](const Request &r) { return npix_of(r) * r.remaining >= fills; };
+    if (!serial && !renders.empty()) {
+        const bool big = is_big(*renders[0]);
+        std::vector<std::shared_ptr<Request>> kind;
+        for (auto &r : renders)
+            if (is_big(*r) == big && (!big || kind.empty())) kind.push_back(r);
+        renders.swap(kind);
+        if (!big) budget = std::max(1.0, budget / 5.0);
+    }
+    for (auto &r : renders) {
+        const uint64_t npix = npix_of(*r);
+        uint32_t q = r->remaining;
+        if (!serial) {
+            if (s->cfg.max_spp_step) q = std::min(q, s->cfg.max_spp_step);
+            const double room = budget - (double)total;
+            if ((double)(npix * q) > room) {
+                const uint32_t fit = (uint32_t)std::max(0.0, std::floor(room / (double)npix));
+                q = plan.empty() ? std::max(1u, std::min(q, fit)) : std::min(q, fit);
+            }
+            if (q == 0) break;
+        }
+        plan.emplace_back(r, q);
+        total += npix * q;
+        if (!serial && (double)total >= budget) break;
+    }
+    uint64_t scratch_off = 0;
+    if (!serial) {
+        while (!plan.empty() && !s->scratch.alloc(16ull * total, scratch_off)) {  // shrink until it fits
+            auto &[r, q] = plan.back();
+            if (plan.size() > 1 || q == 1) {
+                total -= npix_of(*r) * q;
+                plan.pop_back();
+            } else {
+                total -= npix_of(*r) * (q - q / 2);
+                q /= 2;
+            }
+        }
+    }
+    if (plan.empty() && queries.empty()) return false;  // storage is held by steps in flight
     Step step;
     step.start = take_event(s);
     step.end = take_event(s);
+    if (!serial && !plan.empty()) {
+        step.scratch_off = scratch_off;
+        step.scratch_bytes = 16ull * total;
+    }
     std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
     RT_HIP(hipEventRecord(step.start, s->stream));
-    // samples per render this step: an equal share of the step's budget, at least one pass
-    // and at most max_spp_step passes.  A lane runs its pixel's samples of a slice one after
-    // another, so the cap bounds how long the slowest pixel keeps a step's launch alive
-    // after the rest of the device has run dry.
-    const double budget_samples = s->sample_rate * s->cfg.step_us;
-    const double share = budget_samples / (double)std::max<size_t>(1, renders.size());
+    const uint32_t group = (uint32_t)std::max<uint64_t>(1, total / (32ull * device_lanes(s)));
+    char *storage = static_cast<char *>(s->arena_mem.p) + s->scratch_base + scratch_off;
     std::map<uint32_t, std::vector<RenderJob>> by_width;
     LaunchCfg rc;
     bool rc_set = false;
-    for (auto &r : renders) {
-        const uint64_t npix = (uint64_t)r->req.width * r->req.height;
-        // split renders run one sample per lane, so their slice length does not stretch the step
-        const uint32_t cap = r->split ? kSplitSppStep : s->cfg.max_spp_step;
-        uint32_t q = serial ? r->remaining
-                            : (uint32_t)std::min<double>({(double)r->remaining, (double)cap,
-                                                          std::max(1.0, std::floor(share / (double)npix))});
-        q = std::max(1u, std::min(q, r->remaining));
+    for (auto &[r, q] : plan) {
+        const uint64_t npix = npix_of(*r);
         float *accum = reinterpret_cast<float *>(arena_ptr(s, r->offset));
         const uint32_t kflags = r->req.render_flags & ~RT_REQ_OUT_ACCUM;
         if (!rc_set) {
@@ -245,12 +297,8 @@ bool enqueue_step(rt_server *s) {
                           s->segments.as<unsigned long long>(), rc, s->stream);
         } else {
             RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
-            if (r->split) {
-                make_split(j, reinterpret_cast<float4 *>(arena_ptr(s, r->offset + Arena::round(16ull * npix))));
-            } else if (keeps_primaries(r->req)) {
-                j.pcache = reinterpret_cast<uint2 *>(arena_ptr(s, r->offset + Arena::round(16ull * npix)));
-                j.pcache_valid = r->done_units > 0 ? 1u : 0u;  // an earlier slice has stored them
-            }
+            make_split(j, reinterpret_cast<float4 *>(storage), std::min(group, q));
+            storage += 16ull * npix * q;
             by_width[r->scene->width].push_back(j);
         }
         r->remaining -= q;
@@ -334,6 +382,7 @@ void retire_front(rt_server *s, double done_ms) {
         s->stats.completed++;
         s->active.erase(std::find(s->active.begin(), s->active.end(), r));
     }
+    if (step.scratch_bytes) s->scratch.release(step.scratch_off, step.scratch_bytes);
     s->event_pool.push_back(step.start);
     s->event_pool.push_back(step.end);
     s->cv_done.notify_all();
@@ -411,15 +460,16 @@ int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **o
         if (cfg) s->cfg = *cfg;
         if (!s->cfg.arena_bytes) s->cfg.arena_bytes = 4ull << 30;
         if (!s->cfg.max_active) s->cfg.max_active = 64;
-        if (!s->cfg.step_us) s->cfg.step_us = 8000;
+        if (!s->cfg.step_us) s->cfg.step_us = 50000;
         if (!s->cfg.pipeline) s->cfg.pipeline = 2;
         s->cfg.pipeline = std::min(s->cfg.pipeline, 8u);
-        if (!s->cfg.max_spp_step) s->cfg.max_spp_step = 4;
         RT_REQUIRE((s->cfg.flags & ~RT_SERVER_SERIAL) == 0, "unknown server flags");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         RT_HIP(hipSetDevice(ctx->device));
         s->arena_mem.alloc(s->cfg.arena_bytes);
-        s->arena.reset(s->cfg.arena_bytes);
+        s->scratch_base = s->cfg.arena_bytes / 2 / Arena::kAlign * Arena::kAlign;
+        s->arena.reset(s->scratch_base);
+        s->scratch.reset(s->cfg.arena_bytes - s->scratch_base);
         s->segments.alloc(8);
         RT_HIP(hipMemset(s->segments.p, 0, 8));
         // non-blocking: the server's work does not serialise with the legacy default stream
@@ -484,10 +534,7 @@ int rt_server_submit(rt_server *s, const rt_request *req, uint64_t *id) {
             check_stack(*r->scene, resolve_cfg(s->ctx, *r->scene, render ? &s->cfg.render_cfg : &s->cfg.trace_cfg,
                                                !render && (req->trace_flags & RT_TRACE_ANY)));
         }
-        if (req->kind == RT_REQ_RENDER)
-            r->split = !(s->cfg.flags & RT_SERVER_SERIAL) && prefer_split(s->ctx, req->width, req->height, req->spp,
-                                                                         req->render_flags);
-        r->bytes = request_bytes(*req, r->split);
+        r->bytes = request_bytes(*req);
     } catch (const Error &e) {
         std::lock_guard<std::mutex> lk(s->m);
         s->last_error = e.what();
@@ -495,9 +542,14 @@ int rt_server_submit(rt_server *s, const rt_request *req, uint64_t *id) {
     }
     std::lock_guard<std::mutex> lk(s->m);
     if (s->fatal != RT_OK) return s->fatal;
-    if (r->bytes > s->arena.size()) {
-        s->last_error = "request needs " + std::to_string(r->bytes) + " bytes; the arena holds " +
-                        std::to_string(s->arena.size());
+    // the request's own buffers must fit its half of the arena, and one pass over its pixels
+    // must fit the per-sample storage half, so every admitted request can make progress
+    const uint64_t pass = req->kind == RT_REQ_RENDER && !(s->cfg.flags & RT_SERVER_SERIAL)
+                              ? Arena::round(16ull * req->width * req->height) : 0;
+    if (r->bytes > s->arena.size() || pass > s->scratch.size()) {
+        s->last_error = "request needs " + std::to_string(r->bytes) + " + " + std::to_string(pass) +
+                        " bytes; the arena's halves hold " + std::to_string(s->arena.size()) + " + " +
+                        std::to_string(s->scratch.size());
         return RT_ERR_BUDGET;
     }
     if (r->remaining) {  // the scene stays resident until the request completes

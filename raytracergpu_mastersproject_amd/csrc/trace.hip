@@ -269,6 +269,7 @@ LaunchCfg resolve_cfg(const rt_context *ctx, const Scene &sc, const rt_launch_co
     RT_REQUIRE(c.lds_stack == 4 || c.lds_stack == 8 || c.lds_stack == 16, "lds_stack must be 4, 8 or 16");
     c.refill = std::min<uint32_t>(std::max<uint32_t>(c.refill, 1u), 64u);
     if (const char *lw = std::getenv("RTGPU_LEAF_WEIGHT")) c.leaf_weight = (uint32_t)std::max(1, std::atoi(lw));
+    if (const char *sm = std::getenv("RTGPU_SHADE_MIN")) c.shade_min = (uint32_t)std::min(64, std::max(1, std::atoi(sm)));
     return c;
 }
 

@@ -456,7 +456,7 @@ def trace_request(scene_id, rays: np.ndarray, any_hit=False, out: np.ndarray | N
 class Server:
     """A serving loop over a context's resident scenes (rt_server_*)."""
 
-    def __init__(self, ctx: Context, arena_bytes=1 << 30, max_active=64, step_us=8000, pipeline=2, serial=False,
+    def __init__(self, ctx: Context, arena_bytes=1 << 30, max_active=64, step_us=50000, pipeline=2, serial=False,
                  max_spp_step=0, render_cfg: LaunchConfig | None = None, trace_cfg: LaunchConfig | None = None):
         cfg = ServerConfig(arena_bytes, max_active, step_us, pipeline, SERVER_SERIAL if serial else 0, max_spp_step)
         if render_cfg:

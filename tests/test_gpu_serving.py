@@ -145,8 +145,11 @@ def test_server_results_equal_requests_alone(ctx, native, scenes, serial):
     steps (so renders are cut into many sample slices): every result equals the request run alone."""
     ids, params, soup, ico = scenes
     reqs, expect = _mixed_requests(native, ids, params, _rays(ids, (soup, ico)))
-    arena = 48 * 20_000 + 2 * 16 * 128 * 96  # the largest query and two of the largest images
-    srv = native.Server(ctx, arena_bytes=arena, max_active=4, step_us=50, pipeline=2, serial=serial)
+    # the largest query and two of the largest renders (image + one 9-sample slice of split storage)
+    arena = 48 * 20_000 + 2 * 16 * 128 * 96 * 10
+    # at most 2 passes per render per step: renders are cut into slices, several per launch
+    srv = native.Server(ctx, arena_bytes=arena, max_active=4, step_us=2000, pipeline=2, serial=serial,
+                        max_spp_step=2)
     rids = [srv.submit(r) for r in reqs]
     stats = [srv.wait(rid) for rid in rids]
     st = srv.stats()
