@@ -364,9 +364,9 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
             const bool tracing = ps.phase == kTracing;
             const bool on_node = tracing && at_node(t), on_leaf = tracing && at_leaf(t);
             if (choose_leaf_phase(on_node, on_leaf, a.leaf_weight)) {
-                if (on_leaf) leaf_step<S, false>(t, m, rec_of<MULTI>(a, ps));
+                if (on_leaf) leaf_step<S, false, false>(t, m, rec_of<MULTI>(a, ps));
             } else {
-                if (on_node) node_step<W, S, false>(t, m, wide_of<MULTI>(a, ps));
+                if (on_node) node_step<W, S, false, false>(t, m, wide_of<MULTI>(a, ps));
             }
             if (tracing && trav_done(t)) {
                 ps.hslot = t.bslot;

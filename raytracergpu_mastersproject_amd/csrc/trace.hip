@@ -54,7 +54,7 @@ __device__ __forceinline__ void store_hit(const SceneView &sc, const Trav &t, rt
     if (counts) reinterpret_cast<uint2 *>(counts)[i] = make_uint2(t.nbox, t.ntri);
 }
 
-template <int W, int S, bool ANY>
+template <int W, int S, bool ANY, bool COUNT>
 __global__ __launch_bounds__(256) void k_trace_direct(TraceArgs a) {
     extern __shared__ uint2 lds[];
     uint2 spill[kSpillCap];
@@ -66,11 +66,11 @@ __global__ __launch_bounds__(256) void k_trace_direct(TraceArgs a) {
     load_ray(a.rays, i, o, tmin, d, tmax);
     Trav t;
     trav_init(t, o, d, tmin, tmax, a.sc.nwide);
-    trav_run<W, S, ANY>(t, m, a.sc.wide, a.sc.rec, a.leaf_weight);
+    trav_run<W, S, ANY, COUNT>(t, m, a.sc.wide, a.sc.rec, a.leaf_weight);
     store_hit(a.sc, t, a.hits, a.counts, i);
 }
 
-template <int W, int S, bool ANY>
+template <int W, int S, bool ANY, bool COUNT>
 __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
     extern __shared__ uint2 lds[];
     uint2 spill[kSpillCap];
@@ -111,9 +111,9 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
         if (__ballot(rid >= 0) == 0ull) break;  // every lane idle and the queue is drained
         const bool on_node = rid >= 0 && at_node(t), on_leaf = rid >= 0 && at_leaf(t);
         if (choose_leaf_phase(on_node, on_leaf, a.leaf_weight)) {
-            if (on_leaf) leaf_step<S, ANY>(t, m, a.sc.rec);
+            if (on_leaf) leaf_step<S, ANY, COUNT>(t, m, a.sc.rec);
         } else {
-            if (on_node) node_step<W, S, ANY>(t, m, a.sc.wide);
+            if (on_node) node_step<W, S, ANY, COUNT>(t, m, a.sc.wide);
         }
         if (rid >= 0 && trav_done(t)) {
             store_hit(a.sc, t, a.hits, a.counts, (uint32_t)rid);
@@ -195,9 +195,9 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
         if (__ballot(active) == 0ull) break;
         const bool on_node = active && at_node(t), on_leaf = active && at_leaf(t);
         if (choose_leaf_phase(on_node, on_leaf, a.leaf_weight)) {
-            if (on_leaf) leaf_step<S, ANY>(t, m, a.items[item].sc.rec);
+            if (on_leaf) leaf_step<S, ANY, false>(t, m, a.items[item].sc.rec);
         } else {
-            if (on_node) node_step<W, S, ANY>(t, m, a.items[item].sc.wide);
+            if (on_node) node_step<W, S, ANY, false>(t, m, a.items[item].sc.wide);
         }
         if (active && trav_done(t)) {
             store_hit(a.items[item].sc, t, a.items[item].hits, nullptr, local);
@@ -215,30 +215,33 @@ SceneView view_of(const Scene &sc) {
 }
 
 template <int W, int S> struct Kernels {
-    static void *direct(bool any) {
-        return any ? (void *)k_trace_direct<W, S, true> : (void *)k_trace_direct<W, S, false>;
+    template <bool ANY> static void *direct(bool count) {
+        return count ? (void *)k_trace_direct<W, S, ANY, true> : (void *)k_trace_direct<W, S, ANY, false>;
     }
-    static void *persistent(bool any) {
-        return any ? (void *)k_trace_persistent<W, S, true> : (void *)k_trace_persistent<W, S, false>;
+    template <bool ANY> static void *persistent(bool count) {
+        return count ? (void *)k_trace_persistent<W, S, ANY, true> : (void *)k_trace_persistent<W, S, ANY, false>;
     }
-    static void *batch(bool any) {
+    static void *get(int which, bool any, bool count) {
+        if (which == 0) return any ? direct<true>(count) : direct<false>(count);
+        if (which == 1) return any ? persistent<true>(count) : persistent<false>(count);
         return any ? (void *)k_trace_batch<W, S, true> : (void *)k_trace_batch<W, S, false>;
     }
 };
 
-template <int W> void *pick(uint32_t s, int which, bool any) {
+template <int W> void *pick(uint32_t s, int which, bool any, bool count) {
     switch (s) {
-        case 4: return which == 0 ? Kernels<W, 4>::direct(any) : which == 1 ? Kernels<W, 4>::persistent(any) : Kernels<W, 4>::batch(any);
-        case 16: return which == 0 ? Kernels<W, 16>::direct(any) : which == 1 ? Kernels<W, 16>::persistent(any) : Kernels<W, 16>::batch(any);
-        default: return which == 0 ? Kernels<W, 8>::direct(any) : which == 1 ? Kernels<W, 8>::persistent(any) : Kernels<W, 8>::batch(any);
+        case 4: return Kernels<W, 4>::get(which, any, count);
+        case 16: return Kernels<W, 16>::get(which, any, count);
+        default: return Kernels<W, 8>::get(which, any, count);
     }
 }
 
-void *pick_kernel(uint32_t width, uint32_t s, int which, bool any) {
+// which: 0 one lane per ray, 1 persistent, 2 persistent over many scenes (never counts)
+void *pick_kernel(uint32_t width, uint32_t s, int which, bool any, bool count) {
     switch (width) {
-        case 2: return pick<2>(s, which, any);
-        case 8: return pick<8>(s, which, any);
-        default: return pick<4>(s, which, any);
+        case 2: return pick<2>(s, which, any, count);
+        case 8: return pick<8>(s, which, any, count);
+        default: return pick<4>(s, which, any, count);
     }
 }
 
@@ -282,14 +285,14 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
     TraceArgs a{view_of(sc), rays, hits, counts, n, c.refill, nullptr, c.leaf_weight};
     const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
     if (!c.persistent) {
-        void *k = pick_kernel(sc.width, c.lds_stack, 0, any);
+        void *k = pick_kernel(sc.width, c.lds_stack, 0, any, counts != nullptr);
         uint32_t grid = (n + c.block - 1) / c.block;
         void *args[] = {&a};
         RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
     } else {
         a.work = ctx->counter_slot();
         k_zero_counter<<<1, 64, 0, stream>>>(a.work);
-        void *k = pick_kernel(sc.width, c.lds_stack, 1, any);
+        void *k = pick_kernel(sc.width, c.lds_stack, 1, any, counts != nullptr);
         uint32_t grid = persistent_grid(ctx, k, c.block, lds);
         void *args[] = {&a};
         RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
@@ -319,7 +322,7 @@ void launch_trace_items(rt_context *ctx, const std::vector<TraceItem> &items, ui
                     c.leaf_weight};
         const bool any = (flags & RT_TRACE_ANY) != 0;
         const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
-        void *k = pick_kernel(width, c.lds_stack, 2, any);
+        void *k = pick_kernel(width, c.lds_stack, 2, any, false);
         const uint64_t chunks = (total + kChunk - 1) / kChunk, wpb = c.block / 64u;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(
             1, std::min<uint64_t>(persistent_grid(ctx, k, c.block, lds), (chunks + wpb - 1) / wpb));

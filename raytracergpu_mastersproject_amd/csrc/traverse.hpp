@@ -176,7 +176,8 @@ template <int W> __device__ __forceinline__ void sort_keys(uint64_t *k) {
 
 // Process the current wide node.  Precondition: at_node(t).
 //   closest: hit children in increasing (ord_key(tn), entry); ANY: in child order.
-template <int W, int S, bool ANY>
+//   COUNT: keep t.nbox / t.ntri (only launches that report per-ray counts need them).
+template <int W, int S, bool ANY, bool COUNT = true>
 __device__ __forceinline__ void node_step(Trav &t, const StackMem &m, const uint32_t *__restrict__ wide) {
     if (t.cur_tn > t.bt * kRobust) {
         t.have = false;  // culled by a closer hit found since it was pushed
@@ -188,7 +189,7 @@ __device__ __forceinline__ void node_step(Trav &t, const StackMem &m, const uint
 #pragma unroll
         for (int c = 0; c < W; ++c) {
             const bool valid = nd.child[c] != kEmpty;
-            t.nbox += valid ? 1u : 0u;
+            if constexpr (COUNT) t.nbox += valid ? 1u : 0u;
             float x0 = (nd.lo[0][c] - t.o.x) * t.idir.x, x1 = (nd.hi[0][c] - t.o.x) * t.idir.x;
             float y0 = (nd.lo[1][c] - t.o.y) * t.idir.y, y1 = (nd.hi[1][c] - t.o.y) * t.idir.y;
             float z0 = (nd.lo[2][c] - t.o.z) * t.idir.z, z1 = (nd.hi[2][c] - t.o.z) * t.idir.z;
@@ -214,12 +215,12 @@ __device__ __forceinline__ void node_step(Trav &t, const StackMem &m, const uint
 }
 
 // Test the primitives of the current leaf.  Precondition: at_leaf(t).
-template <int S, bool ANY>
+template <int S, bool ANY, bool COUNT = true>
 __device__ __forceinline__ void leaf_step(Trav &t, const StackMem &m, const LeafRecord *__restrict__ rec) {
     if (t.cur_tn <= t.bt * kRobust) {
         const uint32_t first = leaf_first(t.cur), cnt = leaf_count(t.cur);
         for (uint32_t s = first; s < first + cnt; ++s) {
-            t.ntri++;
+            if constexpr (COUNT) t.ntri++;
             const LeafRecord r = rec[s];
             float th = 0.0f, a = 0.0f, b = 0.0f;
             bool hit = is_sphere_record(r) ? sphere_test(r, t.o, t.d, t.tmin, t.bt, th)
@@ -251,16 +252,16 @@ __device__ __forceinline__ bool choose_leaf_phase(bool on_node, bool on_leaf, ui
 }
 
 // Run one ray to completion (used where lanes are not refilled).
-template <int W, int S, bool ANY>
+template <int W, int S, bool ANY, bool COUNT = true>
 __device__ __forceinline__ void trav_run(Trav &t, const StackMem &m, const uint32_t *wide, const LeafRecord *rec,
                                          uint32_t leaf_weight) {
     for (;;) {
         const bool on_node = at_node(t), on_leaf = at_leaf(t);
         if (__ballot(on_node || on_leaf) == 0ull) break;
         if (choose_leaf_phase(on_node, on_leaf, leaf_weight)) {
-            if (on_leaf) leaf_step<S, ANY>(t, m, rec);
+            if (on_leaf) leaf_step<S, ANY, COUNT>(t, m, rec);
         } else {
-            if (on_node) node_step<W, S, ANY>(t, m, wide);
+            if (on_node) node_step<W, S, ANY, COUNT>(t, m, wide);
         }
     }
 }
