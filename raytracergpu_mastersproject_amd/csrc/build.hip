@@ -404,24 +404,87 @@ __global__ void k_records(const rt_triangle *tris, const rt_sphere *sph, const u
     if (i < n) leaf_body(tris, sph, slot_prim, i, nullptr, records, slot_mat);
 }
 
-// ConstructAABBsOfInternalNodes.comp:27-65: second visitor of a node writes the union
-__global__ void k_refit(rt_bvh_node *nodes, const uint32_t *parent, uint32_t *visits, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t id = parent[n - 1 + i];
-    for (uint32_t guard = 0; guard < n; ++guard) {  // a path to the root has fewer than n nodes
-        uint32_t old = __hip_atomic_fetch_add(&visits[id], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old < 1u) return;
-        rt_bvh_node nd = nodes[id];
-        const rt_bvh_node a = nodes[nd.left], b = nodes[nd.right];
-        nd.minX = gmin(a.minX, b.minX); nd.maxX = gmax(a.maxX, b.maxX);
-        nd.minY = gmin(a.minY, b.minY); nd.maxY = gmax(a.maxY, b.maxY);
-        nd.minZ = gmin(a.minZ, b.minZ); nd.maxZ = gmax(a.maxZ, b.maxZ);
-        nodes[id] = nd;
-        __threadfence();
-        if (id == 0) return;
-        id = parent[id];
+// ConstructAABBsOfInternalNodes.comp:27-65 computes each internal box as the union of
+// its children's, bottom-up, with an atomic visit counter per node (the second visitor
+// writes the union).  Here the same unions are computed level by level instead, deepest
+// level first, one launch per level: on this device the eight XCDs keep separate L2
+// caches, so each of the counter scheme's release/acquire pairs writes back and
+// invalidates a whole L2, while launch boundaries make a level's boxes visible for free.
+// Depths come from pointer jumping over the parent links.  In a Karras tree every child's
+// range shares a strictly longer prefix of the 64-bit (code, index) key than its parent's
+// (kdelta below), so the depth is at most 64 and the per-level launches stay few.
+
+// internal node i: jump = parent (the root points at itself), dist = 1 (root 0)
+__global__ void k_depth_init(const uint32_t *parent, uint32_t m, uint32_t *jump, uint32_t *dist) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    jump[i] = i == 0 ? 0u : parent[i];
+    dist[i] = i == 0 ? 0u : 1u;
+}
+
+// one doubling step: dist(i) += dist(jump(i)), jump(i) = jump(jump(i)); *more = 1 (one
+// store per wave) while a jump has not reached the root
+__global__ void k_depth_jump(const uint32_t *jump, const uint32_t *dist, uint32_t *jump2, uint32_t *dist2, uint32_t m,
+                             uint32_t *more) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t j = jump[i];
+    dist2[i] = dist[i] + dist[j];
+    const uint32_t jj = jump[j];
+    jump2[i] = jj;
+    const unsigned long long pending = __ballot(jj != 0u);
+    if (pending != 0ull && __lane_id() == (uint32_t)__ffsll((long long)pending) - 1u) *more = 1u;
+}
+
+// largest depth: block maximum in LDS, one global atomic per block
+__global__ void k_depth_max(const uint32_t *depth, uint32_t m, uint32_t *out) {
+    __shared__ uint32_t best;
+    if (threadIdx.x == 0) best = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) atomicMax(&best, depth[i]);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, best);
+}
+
+// the union for every node at depth d (its children are leaves or deeper nodes)
+__global__ void k_refit_level(rt_bvh_node *nodes, const uint32_t *depth, uint32_t m, uint32_t d) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= m || depth[id] != d) return;
+    rt_bvh_node nd = nodes[id];
+    const rt_bvh_node a = nodes[nd.left], b = nodes[nd.right];
+    nd.minX = gmin(a.minX, b.minX); nd.maxX = gmax(a.maxX, b.maxX);
+    nd.minY = gmin(a.minY, b.minY); nd.maxY = gmax(a.maxY, b.maxY);
+    nd.minZ = gmin(a.minZ, b.minZ); nd.maxZ = gmax(a.maxZ, b.maxZ);
+    nodes[id] = nd;
+}
+
+// Internal boxes of the n-1 internal nodes (leaf boxes are already written).
+void refit_levels(rt_bvh_node *nodes, const uint32_t *parent, uint32_t n, hipStream_t st) {
+    const uint32_t m = n - 1;
+    DevBuf jump(sizeof(uint32_t) * m), dist(sizeof(uint32_t) * m), jump2(sizeof(uint32_t) * m),
+        dist2(sizeof(uint32_t) * m), flag(sizeof(uint32_t));
+    k_depth_init<<<grid_for(m), kB, 0, st>>>(parent, m, jump.as<uint32_t>(), dist.as<uint32_t>());
+    for (uint32_t round = 0; round < 32; ++round) {  // 2^32 exceeds any depth
+        k_fill_u32<<<1, kB, 0, st>>>(flag.as<uint32_t>(), 0u, 1u);
+        k_depth_jump<<<grid_for(m), kB, 0, st>>>(jump.as<uint32_t>(), dist.as<uint32_t>(), jump2.as<uint32_t>(),
+                                                 dist2.as<uint32_t>(), m, flag.as<uint32_t>());
+        std::swap(jump, jump2);
+        std::swap(dist, dist2);
+        uint32_t more = 0;
+        RT_HIP(hipMemcpyAsync(&more, flag.p, sizeof more, hipMemcpyDeviceToHost, st));
+        RT_HIP(hipStreamSynchronize(st));
+        if (!more) break;
     }
+    k_fill_u32<<<1, kB, 0, st>>>(flag.as<uint32_t>(), 0u, 1u);
+    k_depth_max<<<grid_for(m), kB, 0, st>>>(dist.as<uint32_t>(), m, flag.as<uint32_t>());
+    uint32_t deepest = 0;
+    RT_HIP(hipMemcpyAsync(&deepest, flag.p, sizeof deepest, hipMemcpyDeviceToHost, st));
+    RT_HIP(hipStreamSynchronize(st));
+    RT_REQUIRE(deepest < m, "internal: LBVH parent links do not form a tree");
+    for (uint32_t d = deepest + 1; d-- > 0;)
+        k_refit_level<<<grid_for(m), kB, 0, st>>>(nodes, dist.as<uint32_t>(), m, d);
+    RT_HIP(hipStreamSynchronize(st));  // the temporaries are freed on return
 }
 
 // ---------------------------------------------------------------------------
@@ -673,12 +736,7 @@ void build_scene(rt_context *ctx, Scene &sc, const rt_triangle *tris, uint32_t n
         k_leaves<<<grid_for(n), kB, 0, st>>>(dtris.as<rt_triangle>(), dsph.as<rt_sphere>(), sc.slot_prim.as<uint32_t>(), n,
                                              binary.as<rt_bvh_node>() + (n - 1), sc.records.as<LeafRecord>(),
                                              sc.slot_mat.as<uint32_t>());
-        if (n > 1) {
-            DevBuf visits(sizeof(uint32_t) * (n - 1));
-            k_fill_u32<<<grid_for(n - 1), kB, 0, st>>>(visits.as<uint32_t>(), 0u, n - 1);
-            k_refit<<<grid_for(n), kB, 0, st>>>(binary.as<rt_bvh_node>(), parent.as<uint32_t>(), visits.as<uint32_t>(), n);
-            RT_HIP(hipStreamSynchronize(st));
-        }
+        if (n > 1) refit_levels(binary.as<rt_bvh_node>(), parent.as<uint32_t>(), n, st);
         RT_HIP(hipStreamSynchronize(st));
     }
     RT_HIP(hipGetLastError());
