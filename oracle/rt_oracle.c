@@ -607,6 +607,15 @@ static inline uint32_t ord_key(float f) {
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+/* The distance a pushed entry carries: the kernel's stack keeps only the upper 16 bits of
+ * ord_key (traverse.hpp push/pop), which decode to a distance no larger than tn. */
+static inline float stacked_tn(float tn) {
+    uint32_t k = ord_key(tn) & 0xFFFF0000u, b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    float f;
+    memcpy(&f, &b, sizeof f);
+    return f;
+}
+
 static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float tmax, hitrec *best,
                              orc_counts *cnt, int any) {
     const uint32_t W = sc->width;
@@ -673,7 +682,13 @@ static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float 
         } else {
             for (int i = 0; i < nh; ++i) cand[i].tn = -INFINITY; /* any-hit entries are never culled */
         }
-        for (int i = nh - 1; i >= 0; --i) stack[sp++] = cand[i];
+        /* the nearest becomes the current entry at once, with its full distance; the others go
+         * through the kernel's stack */
+        for (int i = nh - 1; i >= 0; --i) {
+            wentry e = cand[i];
+            if (i > 0 && !any) e.tn = stacked_tn(e.tn);
+            stack[sp++] = e;
+        }
         if (sp > 1000) abort();
     }
     return bslot;

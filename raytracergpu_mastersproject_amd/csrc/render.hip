@@ -293,9 +293,9 @@ __device__ __forceinline__ uint32_t find_job(const RenderJob *jobs, uint32_t njo
 
 template <int W, int S, bool MULTI>
 __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
-    extern __shared__ uint2 lds[];
+    extern __shared__ uint32_t lds[];
     uint2 spill[kSpillCap];
-    StackMem m{lds + threadIdx.x, blockDim.x, spill};
+    const StackMem m = stack_mem<S>(lds, spill);
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
     PixelState ps;
@@ -446,7 +446,7 @@ void launch_render_kernel(rt_context *ctx, RenderArgs &a, uint32_t bvh_width, bo
     a.shade_min = c.shade_min;
     a.work = ctx->counter_slot();
     k_zero<<<1, 64, 0, stream>>>(a.work);
-    const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
+    const size_t lds = stack_lds_bytes(c.lds_stack, c.block);
     void *k = pick_render(bvh_width, c.lds_stack, multi);
     int per_cu = 0;
     RT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)c.block, lds));

@@ -56,9 +56,9 @@ __device__ __forceinline__ void store_hit(const SceneView &sc, const Trav &t, rt
 
 template <int W, int S, bool ANY, bool COUNT>
 __global__ __launch_bounds__(256) void k_trace_direct(TraceArgs a) {
-    extern __shared__ uint2 lds[];
+    extern __shared__ uint32_t lds[];
     uint2 spill[kSpillCap];
-    StackMem m{lds + threadIdx.x, blockDim.x, spill};
+    const StackMem m = stack_mem<S>(lds, spill);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     v3 o, d;
@@ -72,9 +72,9 @@ __global__ __launch_bounds__(256) void k_trace_direct(TraceArgs a) {
 
 template <int W, int S, bool ANY, bool COUNT>
 __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
-    extern __shared__ uint2 lds[];
+    extern __shared__ uint32_t lds[];
     uint2 spill[kSpillCap];
-    StackMem m{lds + threadIdx.x, blockDim.x, spill};
+    const StackMem m = stack_mem<S>(lds, spill);
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
     int64_t rid = -1;
@@ -155,9 +155,9 @@ __device__ __forceinline__ uint32_t find_item(const BatchItemDev *items, uint32_
 
 template <int W, int S, bool ANY>
 __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
-    extern __shared__ uint2 lds[];
+    extern __shared__ uint32_t lds[];
     uint2 spill[kSpillCap];
-    StackMem m{lds + threadIdx.x, blockDim.x, spill};
+    const StackMem m = stack_mem<S>(lds, spill);
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
     bool active = false, exhausted = false;
@@ -283,7 +283,7 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
     check_stack(sc, c);
     const bool any = (flags & RT_TRACE_ANY) != 0;
     TraceArgs a{view_of(sc), rays, hits, counts, n, c.refill, nullptr, c.leaf_weight};
-    const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
+    const size_t lds = stack_lds_bytes(c.lds_stack, c.block);
     if (!c.persistent) {
         void *k = pick_kernel(sc.width, c.lds_stack, 0, any, counts != nullptr);
         uint32_t grid = (n + c.block - 1) / c.block;
@@ -321,7 +321,7 @@ void launch_trace_items(rt_context *ctx, const std::vector<TraceItem> &items, ui
         BatchArgs a{static_cast<const BatchItemDev *>(table), (uint32_t)hv.size(), c.refill, total, work,
                     c.leaf_weight};
         const bool any = (flags & RT_TRACE_ANY) != 0;
-        const size_t lds = sizeof(uint2) * c.lds_stack * c.block;
+        const size_t lds = stack_lds_bytes(c.lds_stack, c.block);
         void *k = pick_kernel(width, c.lds_stack, 2, any, false);
         const uint64_t chunks = (total + kChunk - 1) / kChunk, wpb = c.block / 64u;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(

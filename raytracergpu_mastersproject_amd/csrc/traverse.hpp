@@ -12,8 +12,9 @@
 // advances but never the order of its own steps.  This keeps the ~80-instruction
 // triangle test from being executed by the whole wave on almost every iteration.
 //
-// The stack keeps its newest S entries in an LDS column private to the lane and
-// spills older ones to a private array, so deep trees never fault.
+// The stack keeps its newest S entries in LDS columns private to the lane and spills
+// older ones to a private array, so deep trees never fault.  A slot is 6 bytes: the
+// 32-bit entry and the upper half of its distance's order key (see push()).
 //
 // Closest-hit semantics are those of hitBVH (raytraceBVH.comp:195-265) for any
 // traversal order: the smallest t wins and, among equal t, the smallest leaf slot
@@ -78,7 +79,7 @@ __device__ __forceinline__ float ord_dec(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
-constexpr uint32_t kOrdNegInf = 0x007FFFFFu;  // ord_key(-inf): never culled
+constexpr uint32_t kOrdLowest = 0x00800000u;  // ord_key(-FLT_MAX): never culled, and exact in 16 bits
 
 // per-lane traversal state
 struct Trav {
@@ -93,19 +94,40 @@ struct Trav {
     uint32_t nbox, ntri;
 };
 
+// A stack slot holds an entry and the upper 16 bits of ord_key(entry distance).  The
+// truncated key decodes to a distance no larger than the true one, so culling a popped
+// entry against the best hit stays conservative; it may only keep an entry that the full
+// key would have culled.  Six bytes per slot instead of eight let more waves share a
+// CU's LDS.
+constexpr uint32_t kStackSlotBytes = 6;
+
 struct StackMem {
-    uint2 *col;       // this lane's LDS column: entry k at col[k * stride]
+    uint32_t *ent;    // this lane's LDS entry column: slot k at ent[k * stride]
+    uint16_t *key;    // this lane's LDS key column, same indexing
     uint32_t stride;  // block size
-    uint2 *spill;     // private spill array (kSpillCap entries)
+    uint2 *spill;     // private spill array (kSpillCap entries of (entry, key << 16))
 };
 
-// stack entries hold (entry, ord_key(entry distance))
+// LDS layout of a block: S x block entries, then S x block keys
+template <int S> __device__ __forceinline__ StackMem stack_mem(void *lds, uint2 *spill) {
+    uint32_t *ent = static_cast<uint32_t *>(lds);
+    uint16_t *key = reinterpret_cast<uint16_t *>(ent + (size_t)S * blockDim.x);
+    return StackMem{ent + threadIdx.x, key + threadIdx.x, blockDim.x, spill};
+}
+
+inline size_t stack_lds_bytes(uint32_t lds_stack, uint32_t block) {
+    return (size_t)kStackSlotBytes * lds_stack * block;
+}
+
 template <int S> __device__ __forceinline__ void push(Trav &t, const StackMem &m, uint32_t e, uint32_t tnkey) {
     if (t.top - t.nspill == (uint32_t)S) {  // LDS part full: move its oldest entry out
-        if (t.nspill < kSpillCap) m.spill[t.nspill] = m.col[(t.nspill & (S - 1)) * m.stride];
+        const uint32_t o = (t.nspill & (S - 1)) * m.stride;
+        if (t.nspill < kSpillCap) m.spill[t.nspill] = make_uint2(m.ent[o], (uint32_t)m.key[o] << 16);
         t.nspill++;
     }
-    m.col[(t.top & (S - 1)) * m.stride] = make_uint2(e, tnkey);
+    const uint32_t o = (t.top & (S - 1)) * m.stride;
+    m.ent[o] = e;
+    m.key[o] = (uint16_t)(tnkey >> 16);
     t.top++;
 }
 
@@ -113,7 +135,8 @@ template <int S> __device__ __forceinline__ void pop(Trav &t, const StackMem &m)
     uint2 v;
     if (t.top > t.nspill) {
         t.top--;
-        v = m.col[(t.top & (S - 1)) * m.stride];
+        const uint32_t o = (t.top & (S - 1)) * m.stride;
+        v = make_uint2(m.ent[o], (uint32_t)m.key[o] << 16);
     } else {
         t.nspill--;
         t.top--;
@@ -206,7 +229,7 @@ __device__ __forceinline__ void node_step(Trav &t, const StackMem &m, const uint
         } else {
 #pragma unroll
             for (int k = W - 1; k >= 1; --k)
-                if (k < nh) push<S>(t, m, (uint32_t)key[k], ANY ? kOrdNegInf : (uint32_t)(key[k] >> 32));
+                if (k < nh) push<S>(t, m, (uint32_t)key[k], ANY ? kOrdLowest : (uint32_t)(key[k] >> 32));
             t.cur = (uint32_t)key[0];
             t.cur_tn = ANY ? -__uint_as_float(0x7F800000u) : ord_dec((uint32_t)(key[0] >> 32));
         }
