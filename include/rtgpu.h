@@ -171,6 +171,14 @@ int rt_trace(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n,
  * image is the same either way. */
 #define RT_RENDER_SPLIT_SAMPLES 4u       /* always one lane per (pixel, sample) */
 #define RT_RENDER_NO_SPLIT 8u            /* always one lane per pixel */
+/* How paths are scheduled.  An image of at least 2^22 pixel-samples is rendered bounce by
+ * bounce by default: the ray-query kernel traces one segment of every live path from a
+ * queue of rays, then a shading kernel writes the next bounce's queue (primary segments are
+ * traced once per pixel).  Smaller images carry every path to its end inside one persistent
+ * launch.  The image is the same either way.  RT_RENDER_SPLIT_SAMPLES and RT_RENDER_NO_SPLIT
+ * describe the persistent launch and select it. */
+#define RT_RENDER_WAVEFRONT 16u          /* always bounce by bounce */
+#define RT_RENDER_PERSISTENT_PATHS 32u   /* always one persistent launch */
 
 /* Clear an RGBA32F accumulation image to (0,0,0,1) (RaytracerBVH.cpp:750-776). */
 int rt_accum_clear(rt_context *ctx, float *accum, uint32_t npix, void *stream);

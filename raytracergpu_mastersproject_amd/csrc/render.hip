@@ -180,7 +180,61 @@ __device__ __forceinline__ bool end_sample(const RenderArgs &a, PixelState &ps, 
     return false;
 }
 
-// emitted + scatter for a resolved segment (raytraceBVH.comp:94-115, 276-308)
+// One resolved segment of a path (raytraceBVH.comp:94-115, 276-308): adds the emitted
+// light and scatters.  Returns true when the path continues, with (o, d) replaced by the
+// next segment's ray; the caller owns the bounce count.  hslot == kEmpty is a miss.
+__device__ __forceinline__ bool shade_segment(const RenderJob &J, const LeafRecord *rec, uint32_t hslot, float ht,
+                                              v3 &o, v3 &d, uint32_t &st, v3 &color, v3 &atten) {
+    if (hslot == kEmpty) {
+        const v3 background{0.0f, 0.0f, 0.0f};  // _BACKGROUND_COLOR
+        color = color + mul(background, atten);
+        return false;
+    }
+    const LeafRecord r = rec[hslot];
+    const rt_material m = J.mats[J.slot_mat[hslot]];
+    const v3 alb{m.albedo[0], m.albedo[1], m.albedo[2]};
+    const v3 p = o + ht * d;
+    v3 n;
+    if (is_sphere_record(r)) n = (p - v3{r.r0.x, r.r0.y, r.r0.z}) / r.r0.w;
+    else n = v3{r.r1.w, r.r2.w, r.r3.w};
+    const int bf = dot(d, n) > 0.0f ? 1 : 0;
+    const v3 normal = (float)(1 - 2 * bf) * n;
+    const v3 emitted = m.materialType == RT_LIGHT ? alb : v3{0.0f, 0.0f, 0.0f};
+    color = color + mul(emitted, atten);
+    bool scattered = false;
+    v3 att{0.0f, 0.0f, 0.0f}, nd{0.0f, 0.0f, 0.0f};
+    const bool ref_mats = (J.flags & RT_RENDER_REFERENCE_MATERIALS) != 0;
+    if (m.materialType == RT_DIFFUSE) {
+        att = alb;
+        v3 ruv = normalize(random_in_unit_sphere(st));
+        nd = normalize(normal + ruv);
+        scattered = true;
+    } else if (m.materialType == RT_METALLIC && !ref_mats) {
+        att = alb;
+        v3 refl = reflect(d, normal);
+        v3 rs = random_in_unit_sphere(st);
+        nd = normalize(refl + m.albedo[3] * rs);
+        scattered = dot(nd, normal) > 0.0f;
+    } else if (m.materialType == RT_DIELECTRIC && !ref_mats) {
+        att = v3{1.0f, 1.0f, 1.0f};
+        const float ior = m.albedo[3];
+        const float ri = bf ? ior : (1.0f / ior);
+        const float ct = gmin(dot(-d, normal), 1.0f);
+        const float stt = sqrtf(1.0f - ct * ct);
+        const bool cannot = ri * stt > 1.0f;
+        if (cannot || schlick(ct, ri) > rng_next(st)) nd = reflect(d, normal);
+        else nd = refract(d, normal, ri);
+        nd = normalize(nd);
+        scattered = true;
+    }
+    if (!scattered) return false;
+    atten = mul(atten, att);
+    o = p;
+    d = nd;
+    return true;
+}
+
+// a resolved segment of a persistent-kernel path
 template <bool MULTI>
 __device__ __forceinline__ void shade(const RenderArgs &a, PixelState &ps, Trav &t, uint32_t &segs) {
     const RenderJob &J = job_of<MULTI>(a, ps);
@@ -189,56 +243,10 @@ __device__ __forceinline__ void shade(const RenderArgs &a, PixelState &ps, Trav 
         ps.pc_t = ps.ht;
         ps.pc_valid = true;
     }
-    if (ps.hslot == kEmpty) {
-        const v3 background{0.0f, 0.0f, 0.0f};  // _BACKGROUND_COLOR
-        ps.color = ps.color + mul(background, ps.atten);
+    if (!shade_segment(J, rec_of<MULTI>(a, ps), ps.hslot, ps.ht, ps.o, ps.d, ps.st, ps.color, ps.atten)) {
         end_sample<MULTI>(a, ps, t, segs);
         return;
     }
-    const LeafRecord r = rec_of<MULTI>(a, ps)[ps.hslot];
-    const rt_material m = J.mats[J.slot_mat[ps.hslot]];
-    const v3 alb{m.albedo[0], m.albedo[1], m.albedo[2]};
-    const v3 p = ps.o + ps.ht * ps.d;
-    v3 n;
-    if (is_sphere_record(r)) n = (p - v3{r.r0.x, r.r0.y, r.r0.z}) / r.r0.w;
-    else n = v3{r.r1.w, r.r2.w, r.r3.w};
-    const int bf = dot(ps.d, n) > 0.0f ? 1 : 0;
-    const v3 normal = (float)(1 - 2 * bf) * n;
-    const v3 emitted = m.materialType == RT_LIGHT ? alb : v3{0.0f, 0.0f, 0.0f};
-    ps.color = ps.color + mul(emitted, ps.atten);
-    bool scattered = false;
-    v3 att{0.0f, 0.0f, 0.0f}, nd{0.0f, 0.0f, 0.0f};
-    const bool ref_mats = (J.flags & RT_RENDER_REFERENCE_MATERIALS) != 0;
-    if (m.materialType == RT_DIFFUSE) {
-        att = alb;
-        v3 ruv = normalize(random_in_unit_sphere(ps.st));
-        nd = normalize(normal + ruv);
-        scattered = true;
-    } else if (m.materialType == RT_METALLIC && !ref_mats) {
-        att = alb;
-        v3 refl = reflect(ps.d, normal);
-        v3 rs = random_in_unit_sphere(ps.st);
-        nd = normalize(refl + m.albedo[3] * rs);
-        scattered = dot(nd, normal) > 0.0f;
-    } else if (m.materialType == RT_DIELECTRIC && !ref_mats) {
-        att = v3{1.0f, 1.0f, 1.0f};
-        const float ior = m.albedo[3];
-        const float ri = bf ? ior : (1.0f / ior);
-        const float ct = gmin(dot(-ps.d, normal), 1.0f);
-        const float stt = sqrtf(1.0f - ct * ct);
-        const bool cannot = ri * stt > 1.0f;
-        if (cannot || schlick(ct, ri) > rng_next(ps.st)) nd = reflect(ps.d, normal);
-        else nd = refract(ps.d, normal, ri);
-        nd = normalize(nd);
-        scattered = true;
-    }
-    if (!scattered) {
-        end_sample<MULTI>(a, ps, t, segs);
-        return;
-    }
-    ps.atten = mul(ps.atten, att);
-    ps.o = p;
-    ps.d = nd;
     ps.bounce++;
     if (ps.bounce >= J.depth) {
         end_sample<MULTI>(a, ps, t, segs);
@@ -406,6 +414,131 @@ template <bool MULTI> __global__ void k_sum_samples(RenderArgs a) {
     else sum_job(a.job0);
 }
 
+// ---------------------------------------------------------------------------
+// Bounce-by-bounce schedule.  The paths of a pass are its (sample, pixel) pairs, numbered
+// sample-major (p = k * npix + pix), which is also the layout of the pass's sample storage
+// that k_sum_samples adds in order.  Per path the storage holds {colour, carry out} and a
+// second array {attenuation, RNG state}; a queue entry is the segment's ray and its path.
+// Every path performs the persistent kernel's operations in the same order, so images are
+// identical; only the schedule differs.
+// ---------------------------------------------------------------------------
+struct WaveArgs {
+    RenderJob J;
+    uint32_t npix, nsamp;     // pixels; samples of this pass (paths = npix * nsamp)
+    uint32_t prim_per_pixel;  // the first bounce's hits are per pixel (primary cache), else per path
+    uint32_t bounce;          // bounce being shaded (0: primary segments)
+    const rt_ray *rays_in;
+    const uint32_t *ids_in;
+    const rt_hit *hits_in;
+    const uint32_t *n_in;
+    rt_ray *rays_out;
+    uint32_t *ids_out;
+    uint32_t *n_out;
+    float4 *col;  // [path] {colour, carry out}
+    float4 *att;  // [path] {attenuation, RNG state bits}
+    unsigned long long *segments;
+};
+
+__device__ __forceinline__ void store_ray(rt_ray *rays, uint32_t i, v3 o, v3 d) {
+    float4 *q = reinterpret_cast<float4 *>(rays + i);
+    q[0] = make_float4(o.x, o.y, o.z, 0.001f);        // sceneHit's tMin (raytraceBVH.comp:268)
+    q[1] = make_float4(d.x, d.y, d.z, 10000000.0f);   // and tMax (:269)
+}
+
+// primary rays: one per pixel (prim_per_pixel) or one per path, in path order
+__global__ void k_wave_primary(WaveArgs a) {
+    const uint32_t n = a.prim_per_pixel ? a.npix : a.npix * a.nsamp;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t pix = i % a.npix;
+        v3 o, d;
+        primary(a.J, pix % a.J.width, pix / a.J.width, o, d);
+        store_ray(a.rays_out, i, o, d);
+    }
+    if (a.segments && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.segments, (unsigned long long)n);
+}
+
+constexpr uint32_t kShadeBlock = 1024;
+
+// Shades one segment of every live path and queues the continuing ones.  FIRST: bounce 0,
+// one thread per path, which also starts the sample (carry, RNG state, primary ray).
+// Appends to the queue take one atomic per block and round: a block's waves add their
+// counts in LDS first (a counter hit by every wave would serialise them).
+template <bool FIRST> __global__ __launch_bounds__(kShadeBlock) void k_wave_shade(WaveArgs a) {
+    __shared__ uint32_t wave_off[kShadeBlock / 64];
+    __shared__ uint32_t block_base;
+    const RenderJob &J = a.J;
+    const uint32_t n = FIRST ? a.npix * a.nsamp : *a.n_in;
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // every thread of the grid runs the same number of rounds, so the block-wide append below
+    // sees all of its threads; threads past n only take part in it
+    const uint32_t rounds = (n + stride - 1) / stride;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool cont = false;
+        uint32_t p = 0, st = 0;
+        v3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f}, color{0.0f, 0.0f, 0.0f}, atten{1.0f, 1.0f, 1.0f};
+        float next = 0.0f;
+        if (i < n) {
+            uint32_t hslot = kEmpty;
+            float ht = 0.0f;
+            if constexpr (FIRST) {
+                p = i;
+                const uint32_t pix = p % a.npix, k = p / a.npix;
+                const uint32_t x = pix % J.width, y = pix / J.width;
+                const uint32_t seed0 = rng_seed(x, y, J.random_state);
+                const float carry = carry_at(seed0, reinterpret_cast<const float4 *>(J.accum)[pix].w, k);
+                st = seed0 + f2u_sat(carry * 4294967294.0f);  // the literal is 2^32 in fp32
+                next = rng_next(st);
+                primary(J, x, y, o, d);
+                if (J.depth > 0) {
+                    const rt_hit h = a.hits_in[a.prim_per_pixel ? pix : p];
+                    hslot = h.prim;
+                    ht = h.t;
+                }
+            } else {
+                p = a.ids_in[i];
+                const float4 *q = reinterpret_cast<const float4 *>(a.rays_in + i);
+                const float4 r0 = q[0], r1 = q[1];
+                o = v3{r0.x, r0.y, r0.z};
+                d = v3{r1.x, r1.y, r1.z};
+                const float4 c = a.col[p], t = a.att[p];
+                color = v3{c.x, c.y, c.z};
+                next = c.w;
+                atten = v3{t.x, t.y, t.z};
+                st = __float_as_uint(t.w);
+                const rt_hit h = a.hits_in[i];
+                hslot = h.prim;
+                ht = h.t;
+            }
+            cont = shade_segment(J, J.rec, hslot, ht, o, d, st, color, atten) && a.bounce + 1 < J.depth;
+            a.col[p] = make_float4(color.x, color.y, color.z, next);
+            if (cont) a.att[p] = make_float4(atten.x, atten.y, atten.z, __uint_as_float(st));
+        }
+        const unsigned long long live = __ballot(cont);
+        if (lane == 0) wave_off[wid] = (uint32_t)__popcll(live);
+        __syncthreads();
+        if (threadIdx.x == 0) {  // exclusive offsets of the block's waves, one global append
+            uint32_t total = 0;
+            for (uint32_t w = 0; w < blockDim.x / 64u; ++w) {
+                const uint32_t c = wave_off[w];
+                wave_off[w] = total;
+                total += c;
+            }
+            block_base = total ? atomicAdd(a.n_out, total) : 0u;
+            if (total && a.segments) atomicAdd(a.segments, (unsigned long long)total);
+        }
+        __syncthreads();
+        if (cont) {
+            const uint32_t slot = block_base + wave_off[wid] + (uint32_t)__popcll(live & lt);
+            store_ray(a.rays_out, slot, o, d);
+            a.ids_out[slot] = p;
+        }
+        __syncthreads();  // wave_off and block_base are rewritten next round
+    }
+}
+
 __global__ void k_accum_clear(float4 *accum, uint32_t npix) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x)
         accum[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
@@ -545,17 +678,121 @@ RenderJob make_render_job(const Scene &sc, const rt_params &p, uint32_t width, u
     return j;
 }
 
+namespace {
+
+uint64_t wave_paths_per_pass() {
+    uint64_t v = 1ull << 25;
+    if (const char *e = std::getenv("RTGPU_WAVE_PATHS")) v = (uint64_t)std::max(1, std::atoi(e));
+    return v;
+}
+
+}  // namespace
+
+// per path: {colour, carry}, {attenuation, state}, two queues of (ray, path), hits
+uint64_t wave_scratch_bytes(uint64_t paths) { return paths * (16 + 16 + 2 * (32 + 4) + 16) + 256; }
+
+uint64_t per_pass_limit(uint64_t npix) { return std::max<uint64_t>(1, wave_paths_per_pass() / npix); }
+
+// Bounce-by-bounce rendering of one image: passes of whole samples of every pixel, each
+// continuing the carry chain of the previous one.  A pass holds at most wave_paths_per_pass()
+// paths, and no more than `scratch` holds when the caller provides it (else the storage comes
+// from the stream-ordered pool).
+void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, unsigned long long *segments,
+                        const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes) {
+    const uint32_t npix = j.width * j.height;
+    uint64_t per_pass = std::min<uint64_t>(j.spp, per_pass_limit(npix));
+    if (scratch) {
+        RT_REQUIRE(scratch_bytes >= wave_scratch_bytes(npix), "render scratch smaller than one sample of the image");
+        per_pass = std::min<uint64_t>(per_pass, std::max<uint64_t>(1, (scratch_bytes - 256) / (wave_scratch_bytes(npix) - 256)));
+    }
+    const uint64_t maxp = (uint64_t)npix * per_pass;
+    RT_REQUIRE(maxp < (1ull << 32), "too many paths in one pass");
+    LaunchCfg tc = c;
+    tc.persistent = 1;
+    char *mem = scratch;
+    if (!mem) mem = static_cast<char *>(ctx->stream_alloc(wave_scratch_bytes(maxp), stream));
+    float4 *col = reinterpret_cast<float4 *>(mem);
+    float4 *att = col + maxp;
+    rt_ray *rays[2] = {reinterpret_cast<rt_ray *>(att + maxp), nullptr};
+    rays[1] = rays[0] + maxp;
+    rt_hit *hits = reinterpret_cast<rt_hit *>(rays[1] + maxp);
+    uint32_t *ids[2] = {reinterpret_cast<uint32_t *>(hits + maxp), nullptr};
+    ids[1] = ids[0] + maxp;
+    uint32_t *count = ids[1] + maxp;  // two queue counters
+    const bool cache = !(j.flags & RT_RENDER_NO_PRIMARY_CACHE);
+    const uint32_t block = kShadeBlock;
+    // the shading kernels wait on scattered path state and leaf records: many waves per CU
+    const uint32_t full = (uint32_t)ctx->num_cus * 8u;
+    for (uint32_t s0 = 0; s0 < j.spp; s0 += (uint32_t)per_pass) {
+        const uint32_t nsamp = (uint32_t)std::min<uint64_t>(per_pass, j.spp - s0);
+        const uint64_t paths = (uint64_t)npix * nsamp;
+        WaveArgs a{};
+        a.J = j;
+        a.npix = npix;
+        a.nsamp = nsamp;
+        a.prim_per_pixel = cache ? 1u : 0u;
+        a.col = col;
+        a.att = att;
+        a.segments = segments;
+        const auto grid_for = [&](uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + block - 1) / block); };
+        if (j.depth > 0) {
+            const uint64_t nprim = cache ? npix : paths;
+            a.rays_out = rays[0];
+            k_wave_primary<<<grid_for(nprim), block, 0, stream>>>(a);
+            RT_HIP(hipGetLastError());
+            launch_trace_queue(ctx, sc, rays[0], (uint32_t)nprim, nullptr, hits, tc, stream);
+        }
+        int cur = 0;
+        k_zero<<<1, 64, 0, stream>>>(count + cur);
+        a.bounce = 0;
+        a.hits_in = hits;
+        a.rays_out = rays[cur];
+        a.ids_out = ids[cur];
+        a.n_out = count + cur;
+        k_wave_shade<true><<<grid_for(paths), block, 0, stream>>>(a);
+        RT_HIP(hipGetLastError());
+        for (uint32_t b = 1; b < j.depth; ++b) {
+            launch_trace_queue(ctx, sc, rays[cur], (uint32_t)paths, count + cur, hits, tc, stream);
+            k_zero<<<1, 64, 0, stream>>>(count + (cur ^ 1));
+            a.bounce = b;
+            a.rays_in = rays[cur];
+            a.ids_in = ids[cur];
+            a.hits_in = hits;
+            a.n_in = count + cur;
+            a.rays_out = rays[cur ^ 1];
+            a.ids_out = ids[cur ^ 1];
+            a.n_out = count + (cur ^ 1);
+            k_wave_shade<false><<<std::min(full, grid_for(paths)), block, 0, stream>>>(a);
+            RT_HIP(hipGetLastError());
+            cur ^= 1;
+        }
+        RenderArgs sa{};  // the pass's samples, added to the image in order
+        sa.job0 = j;
+        sa.job0.group = 1;
+        sa.job0.samples = col;
+        sa.job0.spp = nsamp;
+        launch_sum_samples(sa, false, npix, stream);
+    }
+    if (!scratch) RT_HIP(hipFreeAsync(mem, stream));
+}
+
 void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,
                    uint32_t flags, float *accum, unsigned long long *segments, const LaunchCfg &c,
                    hipStream_t stream) {
     RenderJob j = make_render_job(sc, p, width, height, spp, flags, accum);
     if (spp == 0) return;
     check_stack(sc, c);
+    const bool layout = (flags & (RT_RENDER_SPLIT_SAMPLES | RT_RENDER_NO_SPLIT | RT_RENDER_PERSISTENT_PATHS)) != 0;
+    RT_REQUIRE(!(layout && (flags & RT_RENDER_WAVEFRONT)), "RT_RENDER_WAVEFRONT excludes the persistent-launch flags");
+    if ((flags & RT_RENDER_WAVEFRONT) || (!layout && (uint64_t)width * height * spp >= (1ull << 22))) {
+        launch_render_wave(ctx, sc, j, segments, c, stream, nullptr, 0);
+        return;
+    }
     float4 *samples = nullptr;
     const uint32_t group = split_group(ctx, width, height, spp, flags);
     const bool split = group != 0;
     if (split) {  // stream-ordered scratch from the device's memory pool, freed after the sum
-        RT_HIP(hipMallocAsync(reinterpret_cast<void **>(&samples), 16ull * width * height * spp, stream));
+        samples = static_cast<float4 *>(ctx->stream_alloc(16ull * width * height * spp, stream));
         make_split(j, samples, group);
     }
     RenderArgs a{};

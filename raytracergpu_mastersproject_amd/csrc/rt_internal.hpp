@@ -115,8 +115,9 @@ struct RenderJob {
 void make_split(RenderJob &j, float4 *samples, uint32_t group);
 // Samples per split item for a whole-image render (0 = one lane per pixel).
 uint32_t split_group(const rt_context *ctx, uint32_t width, uint32_t height, uint32_t spp, uint32_t flags);
-constexpr uint32_t kRenderFlags =
-    RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE | RT_RENDER_SPLIT_SAMPLES | RT_RENDER_NO_SPLIT;
+constexpr uint32_t kRenderFlags = RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE |
+                                   RT_RENDER_SPLIT_SAMPLES | RT_RENDER_NO_SPLIT | RT_RENDER_WAVEFRONT |
+                                   RT_RENDER_PERSISTENT_PATHS;
 constexpr uint32_t kRenderChunk = 64;  // work items (one 8x8 tile) per atomic on the render work counter
 
 // Small host->device tables (batch item lists, render job lists) that a launch reads.
@@ -208,6 +209,17 @@ struct rt_context {
     // Held by every entry point for its duration and by a server's worker thread while
     // it enqueues a step, so a context may be shared by the caller and its servers.
     std::recursive_mutex mu;
+    // Stream-ordered scratch of launches (split samples, path storage).  The pool keeps up to
+    // kPoolKeep bytes reserved across synchronisations, so a render does not map its storage
+    // again after each one (the device's default pool releases everything it holds).
+    hipMemPool_t pool = nullptr;
+    static constexpr uint64_t kPoolKeep = 16ull << 30;
+
+    void *stream_alloc(uint64_t bytes, hipStream_t st) {
+        void *p = nullptr;
+        RT_HIP(hipMallocFromPoolAsync(&p, bytes, pool, st));
+        return p;
+    }
 
     rtgpu::Scene &scene(uint32_t id) {
         auto it = scenes.find(id);
@@ -257,6 +269,10 @@ LaunchCfg resolve_cfg(const rt_context *ctx, const Scene &sc, const rt_launch_co
 void check_stack(const Scene &sc, const LaunchCfg &cfg);
 void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t n, rt_hit *hits,
                   uint32_t *counts, uint32_t flags, const LaunchCfg &cfg, hipStream_t stream);
+// Closest-hit query whose hits carry leaf slots (for shading), over rays[0, min(*n_dev, n_max))
+// when n_dev is set (a count produced on the device), else rays[0, n_max); persistent schedule.
+void launch_trace_queue(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t n_max, const uint32_t *n_dev,
+                        rt_hit *hits, const LaunchCfg &c, hipStream_t stream);
 void launch_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems, uint32_t flags,
                         const LaunchCfg &cfg, hipStream_t stream);
 struct TraceItem {
@@ -271,6 +287,13 @@ void launch_trace_items(rt_context *ctx, const std::vector<TraceItem> &items, ui
 
 // render.hip
 void launch_accum_clear(float *accum, uint32_t npix, hipStream_t stream);
+// Bounce-by-bounce rendering of one job into its image (see RT_RENDER_WAVEFRONT), with the
+// path storage from `scratch` (at least wave_scratch_bytes(width * height)) or, when it is
+// NULL, from the stream-ordered pool.
+uint64_t wave_scratch_bytes(uint64_t paths);
+uint64_t per_pass_limit(uint64_t npix);  // samples of an npix-pixel image per pass
+void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, unsigned long long *segments,
+                        const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes);
 void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height,
                    uint32_t spp, uint32_t flags, float *accum, unsigned long long *segments,
                    const LaunchCfg &cfg, hipStream_t stream);

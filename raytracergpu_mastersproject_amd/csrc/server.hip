@@ -230,8 +230,10 @@ bool enqueue_step(rt_server *s) {
     auto npix_of = [](const Request &r) { return (uint64_t)r.req.width * r.req.height; };
     const uint64_t fills = 32ull * device_lanes(s);
     auto is_big = [&](const Request &r) { return npix_of(r) * r.remaining >= fills; };
+    bool big_step = false;
     if (!serial && !renders.empty()) {
         const bool big = is_big(*renders[0]);
+        big_step = big;
         std::vector<std::shared_ptr<Request>> kind;
         for (auto &r : renders)
             if (is_big(*r) == big && (!big || kind.empty())) kind.push_back(r);
@@ -254,8 +256,30 @@ bool enqueue_step(rt_server *s) {
         total += npix * q;
         if (!serial && (double)total >= budget) break;
     }
-    uint64_t scratch_off = 0;
-    if (!serial) {
+    // A big render alone runs bounce by bounce, like rt_render would run it, unless its
+    // flags ask for the persistent launch; its path storage comes from the scratch half.
+    const uint32_t layout_flags = RT_RENDER_SPLIT_SAMPLES | RT_RENDER_NO_SPLIT | RT_RENDER_PERSISTENT_PATHS;
+    bool wave = big_step && plan.size() == 1 && !(plan[0].first->req.render_flags & layout_flags);
+    uint64_t scratch_off = 0, scratch_bytes = 0;
+    if (wave) {  // halve the slice until its path storage fits; else the persistent launch below
+        auto &[r, q] = plan[0];
+        const uint64_t npix = npix_of(*r);
+        auto need = [&](uint64_t samples) { return wave_scratch_bytes(npix * std::min(samples, per_pass_limit(npix))); };
+        uint32_t qq = q;
+        bool fits = s->scratch.alloc(need(qq), scratch_off);
+        while (!fits && qq > 1) {
+            qq /= 2;
+            fits = s->scratch.alloc(need(qq), scratch_off);
+        }
+        if (fits) {
+            q = qq;
+            total = npix * q;
+            scratch_bytes = need(q);
+        } else {
+            wave = false;
+        }
+    }
+    if (!wave && !serial) {
         while (!plan.empty() && !s->scratch.alloc(16ull * total, scratch_off)) {  // shrink until it fits
             auto &[r, q] = plan.back();
             if (plan.size() > 1 || q == 1) {
@@ -273,7 +297,7 @@ bool enqueue_step(rt_server *s) {
     step.end = take_event(s);
     if (!serial && !plan.empty()) {
         step.scratch_off = scratch_off;
-        step.scratch_bytes = 16ull * total;
+        step.scratch_bytes = wave ? scratch_bytes : 16ull * total;
     }
     std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
     RT_HIP(hipEventRecord(step.start, s->stream));
@@ -295,6 +319,10 @@ bool enqueue_step(rt_server *s) {
         if (serial) {
             launch_render(s->ctx, *r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum,
                           s->segments.as<unsigned long long>(), rc, s->stream);
+        } else if (wave) {
+            RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
+            launch_render_wave(s->ctx, *r->scene, j, s->segments.as<unsigned long long>(), rc, s->stream, storage,
+                               scratch_bytes);
         } else {
             RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
             make_split(j, reinterpret_cast<float4 *>(storage), std::min(group, q));

@@ -26,7 +26,7 @@ struct SceneView {
 };
 
 struct TraceArgs {
-    SceneView sc;
+    SceneView sc;  // sc.slot_prim == nullptr: hits carry the leaf slot instead of the primitive id
     const rt_ray *rays;
     rt_hit *hits;
     uint32_t *counts;
@@ -34,6 +34,7 @@ struct TraceArgs {
     uint32_t refill;
     unsigned int *work;
     uint32_t leaf_weight;
+    const uint32_t *n_dev;  // persistent launches: when set, the ray count is min(*n_dev, n)
 };
 
 __device__ __forceinline__ void load_ray(const rt_ray *rays, uint32_t i, v3 &o, float &tmin, v3 &d, float &tmax) {
@@ -49,7 +50,8 @@ __device__ __forceinline__ void store_hit(const SceneView &sc, const Trav &t, rt
                                           uint32_t i) {
     uint4 h;
     if (t.bslot == kEmpty) h = make_uint4(RT_NO_HIT, __float_as_uint(t.bt), 0u, 0u);
-    else h = make_uint4(sc.slot_prim[t.bslot], __float_as_uint(t.bt), __float_as_uint(t.bu), __float_as_uint(t.bv));
+    else h = make_uint4(sc.slot_prim ? sc.slot_prim[t.bslot] : t.bslot, __float_as_uint(t.bt), __float_as_uint(t.bu),
+                        __float_as_uint(t.bv));
     reinterpret_cast<uint4 *>(hits)[i] = h;
     if (counts) reinterpret_cast<uint2 *>(counts)[i] = make_uint2(t.nbox, t.ntri);
 }
@@ -80,6 +82,11 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
     int64_t rid = -1;
     bool exhausted = false;
     uint32_t q_next = 0, q_end = 0;  // the wave's current chunk of ray indices (wave-uniform)
+    const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;
+    // rays per atomic: kChunk, or fewer when there are too few rays for every wave to take two
+    // chunks (a small launch then ends with its longest ray, not with a wave's run of rays)
+    const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+    const uint32_t chunk = max(64u, min(kChunk, (n / (2u * waves)) & ~63u));
     Trav t;
     t.have = false;
     t.top = 0;
@@ -90,10 +97,10 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
             if (q_next == q_end) {  // one atomic per chunk keeps the shared counter off the critical path
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 uint32_t base = 0;
-                if ((int)lane == leader) base = atomicAdd(a.work, kChunk);
+                if ((int)lane == leader) base = atomicAdd(a.work, chunk);
                 base = (uint32_t)__shfl((int)base, leader);
-                if (base >= a.n) exhausted = true;
-                else { q_next = base; q_end = (uint32_t)min((uint64_t)base + kChunk, (uint64_t)a.n); }
+                if (base >= n) exhausted = true;
+                else { q_next = base; q_end = (uint32_t)min((uint64_t)base + chunk, (uint64_t)n); }
             }
             if (!exhausted) {
                 const uint32_t take = min(nidle, q_end - q_next);
@@ -282,7 +289,7 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
     RT_REQUIRE(rays && hits, "rays/hits must be device pointers");
     check_stack(sc, c);
     const bool any = (flags & RT_TRACE_ANY) != 0;
-    TraceArgs a{view_of(sc), rays, hits, counts, n, c.refill, nullptr, c.leaf_weight};
+    TraceArgs a{view_of(sc), rays, hits, counts, n, c.refill, nullptr, c.leaf_weight, nullptr};
     const size_t lds = stack_lds_bytes(c.lds_stack, c.block);
     if (!c.persistent) {
         void *k = pick_kernel(sc.width, c.lds_stack, 0, any, counts != nullptr);
@@ -297,6 +304,26 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
         void *args[] = {&a};
         RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
     }
+    RT_HIP(hipGetLastError());
+}
+
+void launch_trace_queue(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t n_max, const uint32_t *n_dev,
+                        rt_hit *hits, const LaunchCfg &c, hipStream_t stream) {
+    if (n_max == 0) return;
+    check_stack(sc, c);
+    SceneView v = view_of(sc);
+    v.slot_prim = nullptr;  // hits carry leaf slots
+    TraceArgs a{v, rays, hits, nullptr, n_max, c.refill, ctx->counter_slot(), c.leaf_weight, n_dev};
+    k_zero_counter<<<1, 64, 0, stream>>>(a.work);
+    const size_t lds = stack_lds_bytes(c.lds_stack, c.block);
+    void *k = pick_kernel(sc.width, c.lds_stack, 1, false, false);
+    uint32_t grid = persistent_grid(ctx, k, c.block, lds);
+    if (!n_dev) {  // no more waves than chunks
+        const uint64_t chunks = (n_max + kChunk - 1) / kChunk, wpb = c.block / 64u;
+        grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(grid, (chunks + wpb - 1) / wpb));
+    }
+    void *args[] = {&a};
+    RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
     RT_HIP(hipGetLastError());
 }
 

@@ -34,6 +34,8 @@ RENDER_REFERENCE_MATERIALS = 1
 RENDER_NO_PRIMARY_CACHE = 2
 RENDER_SPLIT_SAMPLES = 4
 RENDER_NO_SPLIT = 8
+RENDER_WAVEFRONT = 16
+RENDER_PERSISTENT_PATHS = 32
 
 
 class BuildOptions(C.Structure):

@@ -145,3 +145,74 @@ def test_depth_zero_and_empty_scene(ctx, native, O):
     g, _ = gpu_render(native, ctx, sid, s["params"], 16, 16, 2)
     assert np.all(g[:, :3] == 0.0)
     ctx.release(sid)
+
+
+@pytest.mark.parametrize("cache", [True, False])
+def test_wavefront_equals_the_oracle(ctx, native, O, cache, monkeypatch):
+    """Bounce-by-bounce rendering (ray-query launches over a queue of segments, then shading)
+    gives the oracle's image bit for bit, in one pass or in passes of 2 samples that continue
+    the carry chain; with the primary cache a pixel's primary segment is traced once per pass."""
+    w, h, spp = 64, 48, 5
+    s = W.icosphere_scene(freq=10, width=w, height=h, spp=spp, depth=6)
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    flags = native.RENDER_WAVEFRONT | (0 if cache else native.RENDER_NO_PRIMARY_CACHE)
+    o, osegs = osc.render(s["params"], w, h, spp, method=O.WIDE)
+    g, segs = gpu_render(native, ctx, sid, s["params"], w, h, spp, flags)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    assert segs == (osegs - w * h * (spp - 1) if cache else osegs)
+    monkeypatch.setenv("RTGPU_WAVE_PATHS", str(2 * w * h))  # passes of 2, 2 and 1 samples
+    g3, segs3 = gpu_render(native, ctx, sid, s["params"], w, h, spp, flags)
+    assert np.array_equal(g3.view(np.uint32), o.view(np.uint32))
+    assert segs3 == (osegs - w * h * (spp - 3) if cache else osegs)
+    # continuing an image: a persistent launch of spp samples, then spp bounce-by-bounce samples
+    g1, _ = gpu_render(native, ctx, sid, s["params"], w, h, spp, native.RENDER_PERSISTENT_PATHS)
+    g2, _ = gpu_render(native, ctx, sid, s["params"], w, h, spp, flags, accum=g1)
+    o2, _ = osc.render(s["params"], w, h, 2 * spp, method=O.WIDE)
+    assert np.array_equal(g2.view(np.uint32), o2.view(np.uint32))
+    ctx.release(sid)
+
+
+def test_wavefront_materials_spheres_and_edges(ctx, native, O):
+    """The bounce-by-bounce schedule on metal/dielectric materials (both material modes), sphere
+    leaves, rows past the seed aliasing at 600, and depth 0 (nothing traced)."""
+    wf = native.RENDER_WAVEFRONT
+    s = _material_scene()
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    for ref in (False, True):
+        g, _ = gpu_render(native, ctx, sid, s["params"], 96, 64, 2,
+                          wf | (native.RENDER_REFERENCE_MATERIALS if ref else 0))
+        o, _ = osc.render(s["params"], 96, 64, 2, flags=O.RENDER_REFERENCE_MATERIALS if ref else 0, method=O.WIDE)
+        assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), ref
+    ctx.release(sid)
+    s = W.soup_scene(ntri=300, width=80, height=60)
+    sph = with_spheres(s["tris"], s["materials"], 6)
+    sid, osc = upload_both(ctx, O, s["tris"], sph, s["materials"])
+    g, _ = gpu_render(native, ctx, sid, s["params"], 80, 60, 2, wf)
+    o, _ = osc.render(s["params"], 80, 60, 2, method=O.WIDE)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    ctx.release(sid)
+    s = W.soup_scene(ntri=128)
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    g, _ = gpu_render(native, ctx, sid, s["params"], 13, 611, 1, wf)
+    o, _ = osc.render(s["params"], 13, 611, 1, method=O.WIDE)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    p = s["params"].copy()
+    p["maxRayTraceDepth"] = 0
+    g, segs = gpu_render(native, ctx, sid, p, 16, 16, 3, wf)
+    o, _ = osc.render(p, 16, 16, 3, method=O.WIDE)
+    assert segs == 0 and np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    with pytest.raises(native.RTError):
+        gpu_render(native, ctx, sid, s["params"], 16, 16, 1, wf | native.RENDER_NO_SPLIT)
+    ctx.release(sid)
+
+
+def test_default_schedule_of_a_large_image(ctx, native, O):
+    """An image of at least 2^22 pixel-samples is rendered bounce by bounce by default, with the
+    same image as the persistent launch (which the tests above pin to the oracle)."""
+    s = W.icosphere_scene(freq=16, width=512, height=512, spp=16, depth=8)
+    sid = ctx.upload(s["tris"], None, s["materials"])
+    gd, segd = gpu_render(native, ctx, sid, s["params"], 512, 512, 16)
+    gp, segp = gpu_render(native, ctx, sid, s["params"], 512, 512, 16, native.RENDER_PERSISTENT_PATHS)
+    assert np.array_equal(gd.view(np.uint32), gp.view(np.uint32))
+    assert 0 < segd < segp  # primaries once per pixel instead of once per work item
+    ctx.release(sid)

@@ -280,3 +280,24 @@ def test_served_scene_requests_through_the_server(ctx, native):
     assert any(o[3][:, :3].max() > 0 for o in outs)  # the emitter is reached
     for sid in sids:
         ctx.release(sid)
+
+
+def test_big_request_runs_bounce_by_bounce(ctx, native, scenes):
+    """A render that fills the device alone runs bounce by bounce with its path storage in the
+    server's arena; a tight arena cuts it into smaller slices.  The image is the persistent
+    launch's in every case."""
+    ids, params, _, _ = scenes
+    w, h, spp = 512, 512, 40  # 10.5 M pixel-samples: more than ~32 per resident lane
+    ref = native.DeviceArray(16 * w * h)
+    ctx.accum_clear(ref, w * h)
+    ctx.render(ids["ico"], params["ico"], w, h, spp, ref, flags=native.RENDER_PERSISTENT_PATHS)
+    expect = ref.to_numpy(np.float32, 4 * w * h)
+    # roomy: the whole request in one slice; tight: the scratch half holds ~4 samples of paths
+    for arena in (4 << 30, 2 * (16 * w * h + 4 * 120 * w * h) + (1 << 20)):
+        out = np.zeros((w * h, 4), np.float32)
+        srv = native.Server(ctx, arena_bytes=arena, step_us=200000)
+        st = srv.wait(srv.submit(native.render_request(ids["ico"], params["ico"], w, h, spp, native.REQ_OUT_ACCUM,
+                                                       out=out)))
+        srv.close()
+        assert st.status == 0
+        assert np.array_equal(out.reshape(-1).view(np.uint32), expect.view(np.uint32))
