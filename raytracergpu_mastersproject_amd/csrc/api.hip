@@ -55,19 +55,7 @@ int rt_create(rt_context **out, int device) {
         RT_HIP(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device));
         ctx->counters.alloc(16ull * rt_context::kCounterSlots);
         RT_HIP(hipMemset(ctx->counters.p, 0, ctx->counters.bytes));
-        hipMemPoolProps props{};
-        props.allocType = hipMemAllocationTypePinned;
-        props.handleTypes = hipMemHandleTypeNone;
-        props.location.type = hipMemLocationTypeDevice;
-        props.location.id = device;
-        RT_HIP(hipMemPoolCreate(&ctx->pool, &props));
-        uint64_t keep = rt_context::kPoolKeep;
-        RT_HIP(hipMemPoolSetAttribute(ctx->pool, hipMemPoolAttrReleaseThreshold, &keep));
     });
-    if (st != RT_OK && ctx->pool) {
-        (void)hipMemPoolDestroy(ctx->pool);
-        ctx->pool = nullptr;
-    }
     if (st != RT_OK) {
         delete ctx;
         return st;
@@ -80,7 +68,6 @@ void rt_destroy(rt_context *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
-    if (ctx->pool) (void)hipMemPoolDestroy(ctx->pool);
     delete ctx;
 }
 
@@ -221,15 +208,15 @@ int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, 
                     split.emplace_back(&rj, g);
                     split_bytes += 16ull * rj.width * rj.height * rj.spp;
                 }
-        char *scratch = nullptr;  // one stream-ordered allocation for every split job's samples
-        if (split_bytes) scratch = static_cast<char *>(ctx->stream_alloc(split_bytes, st));
+        std::unique_ptr<ScratchLease> lease;  // one block for every split job's samples
+        if (split_bytes) lease = std::make_unique<ScratchLease>(ctx->scratch, split_bytes, st);
+        char *scratch = lease ? lease->ptr() : nullptr;
         uint64_t off = 0;
         for (auto &[rj, g] : split) {
             make_split(*rj, reinterpret_cast<float4 *>(scratch + off), g);
             off += 16ull * rj->width * rj->height * rj->spp;
         }
         for (auto &kv : by_width) launch_render_jobs(ctx, kv.second, kv.first, segments, c, st);
-        if (scratch) RT_HIP(hipFreeAsync(scratch, st));
     });
 }
 

@@ -539,6 +539,116 @@ template <bool FIRST> __global__ __launch_bounds__(kShadeBlock) void k_wave_shad
     }
 }
 
+// Runs the paths of a queue to their ends in one persistent launch.  The late bounces hold
+// a small share of the paths, mostly rays that leave a surface inside a closed mesh and are
+// long to trace; bounce-by-bounce launches would each wait for their longest ray, while
+// here a lane continues its own path as soon as its segment is shaded.  Lanes take queue
+// entries like the ray-query kernel takes rays, and shade in phases like k_render.
+struct FinishArgs {
+    WaveArgs w;
+    unsigned int *work;
+    uint32_t refill, leaf_weight, shade_min;
+};
+
+template <int W, int S> __global__ __launch_bounds__(256) void k_wave_finish(FinishArgs f) {
+    extern __shared__ uint32_t lds[];
+    uint2 spill[kSpillCap];
+    const StackMem m = stack_mem<S>(lds, spill);
+    const WaveArgs &a = f.w;
+    const RenderJob &J = a.J;
+    const uint32_t n = *a.n_in;
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
+    uint32_t phase = kIdle, p = 0, bounce = 0, st = 0, hslot = kEmpty;
+    float next = 0.0f, ht = 0.0f;
+    v3 color{0.0f, 0.0f, 0.0f}, atten{1.0f, 1.0f, 1.0f};
+    Trav t;
+    t.have = false;
+    t.top = 0;
+    uint32_t segs = 0;  // continuation segments (the queue's own were counted when queued)
+    bool exhausted = false;
+    uint32_t q_next = 0, q_end = 0;
+    for (;;) {
+        const unsigned long long idle = __ballot(phase == kIdle);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && nidle >= f.refill) {
+            if (q_next == q_end) {
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(f.work, kRenderChunk);
+                base = (uint32_t)__shfl((int)base, leader);
+                if (base >= n) exhausted = true;
+                else { q_next = base; q_end = min(base + kRenderChunk, n); }
+            }
+            if (!exhausted) {
+                const uint32_t take = min(nidle, q_end - q_next);
+                const uint32_t r = (uint32_t)__popcll(idle & lt);
+                if (phase == kIdle && r < take) {
+                    const uint32_t i = q_next + r;
+                    p = a.ids_in[i];
+                    const float4 *q = reinterpret_cast<const float4 *>(a.rays_in + i);
+                    const float4 r0 = q[0], r1 = q[1];
+                    const float4 c = a.col[p], at = a.att[p];
+                    color = v3{c.x, c.y, c.z};
+                    next = c.w;
+                    atten = v3{at.x, at.y, at.z};
+                    st = __float_as_uint(at.w);
+                    bounce = a.bounce;
+                    trav_init(t, v3{r0.x, r0.y, r0.z}, v3{r1.x, r1.y, r1.z}, r0.w, r1.w, J.nwide);
+                    phase = kTracing;
+                }
+                q_next += take;
+            }
+        }
+        if (__ballot(phase != kIdle) == 0ull) break;
+        const unsigned long long mres = __ballot(phase == kResolved), mtr = __ballot(phase == kTracing);
+        if (mres != 0ull && (mtr == 0ull || (uint32_t)__popcll(mres) >= f.shade_min)) {
+            if (phase == kResolved) {
+                v3 o = t.o, d = t.d;
+                if (shade_segment(J, J.rec, hslot, ht, o, d, st, color, atten) && bounce + 1 < J.depth) {
+                    bounce++;
+                    trav_init(t, o, d, 0.001f, 10000000.0f, J.nwide);
+                    phase = kTracing;
+                    segs++;
+                } else {
+                    a.col[p] = make_float4(color.x, color.y, color.z, next);
+                    phase = kIdle;
+                }
+            }
+        } else {
+            const bool tracing = phase == kTracing;
+            const bool on_node = tracing && at_node(t), on_leaf = tracing && at_leaf(t);
+            if (choose_leaf_phase(on_node, on_leaf, f.leaf_weight)) {
+                if (on_leaf) leaf_step<S, false, false>(t, m, J.rec);
+            } else {
+                if (on_node) node_step<W, S, false, false>(t, m, J.wide);
+            }
+            if (tracing && trav_done(t)) {
+                hslot = t.bslot;
+                ht = t.bt;
+                phase = kResolved;
+            }
+        }
+    }
+    if (a.segments) {
+        unsigned long long sg = segs;
+        for (int off = 32; off > 0; off >>= 1) sg += __shfl_xor(sg, off);
+        if (lane == 0 && sg) atomicAdd(a.segments, sg);
+    }
+}
+
+template <int W> void *pick_finish_s(uint32_t s) {
+    return s == 4 ? (void *)k_wave_finish<W, 4> : s == 16 ? (void *)k_wave_finish<W, 16> : (void *)k_wave_finish<W, 8>;
+}
+
+void *pick_finish(uint32_t width, uint32_t s) {
+    switch (width) {
+        case 2: return pick_finish_s<2>(s);
+        case 8: return pick_finish_s<8>(s);
+        default: return pick_finish_s<4>(s);
+    }
+}
+
 __global__ void k_accum_clear(float4 *accum, uint32_t npix) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x)
         accum[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
@@ -693,6 +803,17 @@ uint64_t wave_scratch_bytes(uint64_t paths) { return paths * (16 + 16 + 2 * (32 
 
 uint64_t per_pass_limit(uint64_t npix) { return std::max<uint64_t>(1, wave_paths_per_pass() / npix); }
 
+namespace {
+// First bounce whose segments k_wave_finish traces; the earlier ones go bounce by bounce.
+// Bounce 0 (primary segments, traced once per pixel, and every path's first shading) is the
+// bulk of the work and fills the device; from bounce 1 on, paths thin out and lengthen.
+uint32_t wave_finish_bounce() {
+    uint32_t v = 1;
+    if (const char *e = std::getenv("RTGPU_WAVE_FINISH")) v = (uint32_t)std::max(1, std::atoi(e));
+    return v;
+}
+}  // namespace
+
 // Bounce-by-bounce rendering of one image: passes of whole samples of every pixel, each
 // continuing the carry chain of the previous one.  A pass holds at most wave_paths_per_pass()
 // paths, and no more than `scratch` holds when the caller provides it (else the storage comes
@@ -709,8 +830,9 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
     RT_REQUIRE(maxp < (1ull << 32), "too many paths in one pass");
     LaunchCfg tc = c;
     tc.persistent = 1;
-    char *mem = scratch;
-    if (!mem) mem = static_cast<char *>(ctx->stream_alloc(wave_scratch_bytes(maxp), stream));
+    std::unique_ptr<ScratchLease> lease;
+    if (!scratch) lease = std::make_unique<ScratchLease>(ctx->scratch, wave_scratch_bytes(maxp), stream);
+    char *mem = scratch ? scratch : lease->ptr();
     float4 *col = reinterpret_cast<float4 *>(mem);
     float4 *att = col + maxp;
     rt_ray *rays[2] = {reinterpret_cast<rt_ray *>(att + maxp), nullptr};
@@ -718,7 +840,7 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
     rt_hit *hits = reinterpret_cast<rt_hit *>(rays[1] + maxp);
     uint32_t *ids[2] = {reinterpret_cast<uint32_t *>(hits + maxp), nullptr};
     ids[1] = ids[0] + maxp;
-    uint32_t *count = ids[1] + maxp;  // two queue counters
+    uint32_t *count = ids[1] + maxp;  // two queue counters and a work counter
     const bool cache = !(j.flags & RT_RENDER_NO_PRIMARY_CACHE);
     const uint32_t block = kShadeBlock;
     // the shading kernels wait on scattered path state and leaf records: many waves per CU
@@ -751,7 +873,25 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
         a.n_out = count + cur;
         k_wave_shade<true><<<grid_for(paths), block, 0, stream>>>(a);
         RT_HIP(hipGetLastError());
+        const uint32_t finish_at = wave_finish_bounce();
         for (uint32_t b = 1; b < j.depth; ++b) {
+            if (b >= finish_at) {  // the remaining bounces of the queued paths in one launch
+                a.bounce = b;
+                a.rays_in = rays[cur];
+                a.ids_in = ids[cur];
+                a.n_in = count + cur;
+                FinishArgs fa{a, count + 2, tc.refill, tc.leaf_weight, tc.shade_min};
+                k_zero<<<1, 64, 0, stream>>>(fa.work);
+                const size_t lds = stack_lds_bytes(tc.lds_stack, tc.block);
+                void *k = pick_finish(sc.width, tc.lds_stack);
+                int per_cu = 0;
+                RT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)tc.block, lds));
+                void *args[] = {&fa};
+                RT_HIP(hipLaunchKernel(k, dim3((uint32_t)std::max(1, per_cu) * (uint32_t)ctx->num_cus), dim3(tc.block),
+                                       args, lds, stream));
+                RT_HIP(hipGetLastError());
+                break;
+            }
             launch_trace_queue(ctx, sc, rays[cur], (uint32_t)paths, count + cur, hits, tc, stream);
             k_zero<<<1, 64, 0, stream>>>(count + (cur ^ 1));
             a.bounce = b;
@@ -773,7 +913,6 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
         sa.job0.spp = nsamp;
         launch_sum_samples(sa, false, npix, stream);
     }
-    if (!scratch) RT_HIP(hipFreeAsync(mem, stream));
 }
 
 void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,
@@ -791,8 +930,10 @@ void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_
     float4 *samples = nullptr;
     const uint32_t group = split_group(ctx, width, height, spp, flags);
     const bool split = group != 0;
-    if (split) {  // stream-ordered scratch from the device's memory pool, freed after the sum
-        samples = static_cast<float4 *>(ctx->stream_alloc(16ull * width * height * spp, stream));
+    std::unique_ptr<ScratchLease> lease;  // the samples' storage, back in the cache after the sum
+    if (split) {
+        lease = std::make_unique<ScratchLease>(ctx->scratch, 16ull * width * height * spp, stream);
+        samples = reinterpret_cast<float4 *>(lease->ptr());
         make_split(j, samples, group);
     }
     RenderArgs a{};
@@ -802,10 +943,7 @@ void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_
     a.nitems = j.nitems;
     a.segments = segments;
     launch_render_kernel(ctx, a, sc.width, false, c, stream);
-    if (split) {
-        launch_sum_samples(a, false, width * height, stream);
-        RT_HIP(hipFreeAsync(samples, stream));
-    }
+    if (split) launch_sum_samples(a, false, width * height, stream);
 }
 
 uint64_t launch_render_jobs(rt_context *ctx, std::vector<RenderJob> &jobs, uint32_t bvh_width,

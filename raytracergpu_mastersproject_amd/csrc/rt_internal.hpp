@@ -192,6 +192,97 @@ public:
 
 }  // namespace rtgpu
 
+namespace rtgpu {
+
+// Device scratch of launches (split samples, path storage), kept for reuse.  A block is
+// taken for the host-side enqueue of one launch sequence and given back with an event
+// recorded on its stream after its last use: the same stream may take it again at once
+// (stream order), another stream once the event has completed.  New blocks are allocated
+// only when no free block is large enough; completed free blocks beyond kKeep bytes are
+// freed.  The stream-ordered allocator's pools were avoided: with multi-GiB requests of
+// varying size they stalled single launches for hundreds of milliseconds.
+class ScratchCache {
+  public:
+    static constexpr uint64_t kKeep = 16ull << 30;
+    static constexpr uint64_t kGrain = 64ull << 20;
+
+    struct Block {
+        DevBuf buf;
+        hipEvent_t done = nullptr;
+        hipStream_t last = nullptr;  // stream of the last use (valid when used)
+        bool used = false, taken = false;
+    };
+
+    ~ScratchCache() {
+        for (auto &b : blocks_)
+            if (b->done) (void)hipEventDestroy(b->done);
+    }
+
+    Block *take(uint64_t bytes, hipStream_t st) {
+        Block *best = nullptr;
+        for (auto &b : blocks_) {
+            if (b->taken || b->buf.bytes < bytes || !usable(*b, st)) continue;
+            if (!best || b->buf.bytes < best->buf.bytes) best = b.get();
+        }
+        if (!best) {
+            trim(bytes);
+            auto b = std::make_unique<Block>();
+            b->buf.alloc((bytes + kGrain - 1) / kGrain * kGrain);
+            RT_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+            best = b.get();
+            blocks_.push_back(std::move(b));
+        }
+        best->taken = true;
+        return best;
+    }
+
+    void give(Block *b, hipStream_t st) {
+        RT_HIP(hipEventRecord(b->done, st));
+        b->last = st;
+        b->used = true;
+        b->taken = false;
+    }
+
+  private:
+    std::vector<std::unique_ptr<Block>> blocks_;
+
+    static bool usable(const Block &b, hipStream_t st) {
+        return !b.used || b.last == st || hipEventQuery(b.done) == hipSuccess;
+    }
+    // frees completed free blocks while the cache and the coming block exceed kKeep
+    void trim(uint64_t coming) {
+        uint64_t held = coming;
+        for (auto &b : blocks_) held += b->buf.bytes;
+        for (auto it = blocks_.begin(); it != blocks_.end() && held > kKeep;) {
+            Block &b = **it;
+            if (!b.taken && (!b.used || hipEventQuery(b.done) == hipSuccess)) {
+                held -= b.buf.bytes;
+                if (b.done) (void)hipEventDestroy(b.done);
+                it = blocks_.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+};
+
+// A cache block held for one scope of enqueues.
+class ScratchLease {
+  public:
+    ScratchLease(ScratchCache &c, uint64_t bytes, hipStream_t st) : c_(c), st_(st), b_(c.take(bytes, st)) {}
+    ~ScratchLease() { c_.give(b_, st_); }
+    ScratchLease(const ScratchLease &) = delete;
+    ScratchLease &operator=(const ScratchLease &) = delete;
+    char *ptr() const { return b_->buf.as<char>(); }
+
+  private:
+    ScratchCache &c_;
+    hipStream_t st_;
+    ScratchCache::Block *b_;
+};
+
+}  // namespace rtgpu
+
 struct rt_context {
     int device = 0;
     std::string last_error;
@@ -209,17 +300,7 @@ struct rt_context {
     // Held by every entry point for its duration and by a server's worker thread while
     // it enqueues a step, so a context may be shared by the caller and its servers.
     std::recursive_mutex mu;
-    // Stream-ordered scratch of launches (split samples, path storage).  The pool keeps up to
-    // kPoolKeep bytes reserved across synchronisations, so a render does not map its storage
-    // again after each one (the device's default pool releases everything it holds).
-    hipMemPool_t pool = nullptr;
-    static constexpr uint64_t kPoolKeep = 16ull << 30;
-
-    void *stream_alloc(uint64_t bytes, hipStream_t st) {
-        void *p = nullptr;
-        RT_HIP(hipMallocFromPoolAsync(&p, bytes, pool, st));
-        return p;
-    }
+    rtgpu::ScratchCache scratch;  // launch scratch (split samples, path storage)
 
     rtgpu::Scene &scene(uint32_t id) {
         auto it = scenes.find(id);

@@ -318,8 +318,8 @@ void launch_trace_queue(rt_context *ctx, const Scene &sc, const rt_ray *rays, ui
     const size_t lds = stack_lds_bytes(c.lds_stack, c.block);
     void *k = pick_kernel(sc.width, c.lds_stack, 1, false, false);
     uint32_t grid = persistent_grid(ctx, k, c.block, lds);
-    if (!n_dev) {  // no more waves than chunks
-        const uint64_t chunks = (n_max + kChunk - 1) / kChunk, wpb = c.block / 64u;
+    if (!n_dev) {  // no more waves than chunks of the smallest size the kernel takes
+        const uint64_t chunks = (n_max + 63) / 64, wpb = c.block / 64u;
         grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(grid, (chunks + wpb - 1) / wpb));
     }
     void *args[] = {&a};
