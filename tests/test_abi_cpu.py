@@ -153,3 +153,15 @@ def test_server_entry_points_reject_null_handles():
     assert N.lib.rt_server_destroy(None) == 1
     assert N.lib.rt_server_last_error(None) == b"null server"
     assert N.lib.rt_render_batch(None, None, 0, None, None, None) == 1
+
+
+def test_flag_constants_match_the_c_header():
+    """Every RT_BUILD_/RT_TRACE_/RT_RENDER_/RT_REQ_/RT_SERVER_ #define has the same value in the binding."""
+    import re
+    from raytracergpu_mastersproject_amd import native as N
+    text = (ROOT / "include" / "rtgpu.h").read_text()
+    found = 0
+    for name, value in re.findall(r"^#define RT_((?:BUILD|TRACE|RENDER|REQ|SERVER)_\w+) (0x[0-9A-Fa-f]+|\d+)u", text, re.M):
+        assert getattr(N, name) == int(value, 0), name
+        found += 1
+    assert found >= 15
