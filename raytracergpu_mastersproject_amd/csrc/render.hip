@@ -459,83 +459,112 @@ __global__ void k_wave_primary(WaveArgs a) {
 
 constexpr uint32_t kShadeBlock = 1024;
 
-// Shades one segment of every live path and queues the continuing ones.  FIRST: bounce 0,
-// one thread per path, which also starts the sample (carry, RNG state, primary ray).
-// Appends to the queue take one atomic per block and round: a block's waves add their
-// counts in LDS first (a counter hit by every wave would serialise them).
-template <bool FIRST> __global__ __launch_bounds__(kShadeBlock) void k_wave_shade(WaveArgs a) {
-    __shared__ uint32_t wave_off[kShadeBlock / 64];
-    __shared__ uint32_t block_base;
-    const RenderJob &J = a.J;
-    const uint32_t n = FIRST ? a.npix * a.nsamp : *a.n_in;
+// Appends the block's continuing paths to the queue with one atomic: the block's waves add
+// their counts in LDS first (a counter hit by every wave would serialise them).  Every
+// thread of the block calls it.
+struct BlockAppend {
+    uint32_t wave_off[kShadeBlock / 64];
+    uint32_t base;
+};
+
+__device__ __forceinline__ void append_block(BlockAppend &sh, const WaveArgs &a, bool cont, v3 o, v3 d, uint32_t p) {
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
+    const unsigned long long live = __ballot(cont);
+    if (lane == 0) sh.wave_off[wid] = (uint32_t)__popcll(live);
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive offsets of the block's waves, one global append
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64u; ++w) {
+            const uint32_t c = sh.wave_off[w];
+            sh.wave_off[w] = total;
+            total += c;
+        }
+        sh.base = total ? atomicAdd(a.n_out, total) : 0u;
+        if (total && a.segments) atomicAdd(a.segments, (unsigned long long)total);
+    }
+    __syncthreads();
+    if (cont) {
+        const uint32_t slot = sh.base + sh.wave_off[wid] + (uint32_t)__popcll(live & lt);
+        store_ray(a.rays_out, slot, o, d);
+        a.ids_out[slot] = p;
+    }
+    __syncthreads();  // the offsets are rewritten by the next call
+}
+
+// Bounce 0 of a pass, one thread per pixel: its samples in order, each starting from the carry
+// the previous one left (raytraceBVH.comp:353-375), from the pixel's primary ray and its
+// primary hit; the paths that scatter are queued.
+__global__ __launch_bounds__(kShadeBlock) void k_wave_first(WaveArgs a) {
+    __shared__ BlockAppend sh;
+    const RenderJob &J = a.J;
     const uint32_t stride = gridDim.x * blockDim.x;
-    // every thread of the grid runs the same number of rounds, so the block-wide append below
-    // sees all of its threads; threads past n only take part in it
-    const uint32_t rounds = (n + stride - 1) / stride;
+    const uint32_t rounds = (a.npix + stride - 1) / stride;  // the same for every thread of the grid
     for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
-        bool cont = false;
-        uint32_t p = 0, st = 0;
-        v3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f}, color{0.0f, 0.0f, 0.0f}, atten{1.0f, 1.0f, 1.0f};
-        float next = 0.0f;
-        if (i < n) {
-            uint32_t hslot = kEmpty;
-            float ht = 0.0f;
-            if constexpr (FIRST) {
-                p = i;
-                const uint32_t pix = p % a.npix, k = p / a.npix;
-                const uint32_t x = pix % J.width, y = pix / J.width;
-                const uint32_t seed0 = rng_seed(x, y, J.random_state);
-                const float carry = carry_at(seed0, reinterpret_cast<const float4 *>(J.accum)[pix].w, k);
-                st = seed0 + f2u_sat(carry * 4294967294.0f);  // the literal is 2^32 in fp32
-                next = rng_next(st);
-                primary(J, x, y, o, d);
+        const uint32_t pix = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        const bool live = pix < a.npix;
+        uint32_t x = 0, y = 0, seed0 = 0;
+        float carry = 0.0f;
+        v3 o0{0.0f, 0.0f, 0.0f}, d0{0.0f, 0.0f, 0.0f};
+        if (live) {
+            x = pix % J.width;
+            y = pix / J.width;
+            seed0 = rng_seed(x, y, J.random_state);
+            carry = reinterpret_cast<const float4 *>(J.accum)[pix].w;
+            primary(J, x, y, o0, d0);
+        }
+        for (uint32_t k = 0; k < a.nsamp; ++k) {
+            const uint32_t p = k * a.npix + pix;
+            bool cont = false;
+            v3 o = o0, d = d0;
+            if (live) {
+                uint32_t st = seed0 + f2u_sat(carry * 4294967294.0f);  // the literal is 2^32 in fp32
+                const float next = rng_next(st);
+                carry = next;
+                v3 color{0.0f, 0.0f, 0.0f}, atten{1.0f, 1.0f, 1.0f};
+                uint32_t hslot = kEmpty;
+                float ht = 0.0f;
                 if (J.depth > 0) {
                     const rt_hit h = a.hits_in[a.prim_per_pixel ? pix : p];
                     hslot = h.prim;
                     ht = h.t;
                 }
-            } else {
-                p = a.ids_in[i];
-                const float4 *q = reinterpret_cast<const float4 *>(a.rays_in + i);
-                const float4 r0 = q[0], r1 = q[1];
-                o = v3{r0.x, r0.y, r0.z};
-                d = v3{r1.x, r1.y, r1.z};
-                const float4 c = a.col[p], t = a.att[p];
-                color = v3{c.x, c.y, c.z};
-                next = c.w;
-                atten = v3{t.x, t.y, t.z};
-                st = __float_as_uint(t.w);
-                const rt_hit h = a.hits_in[i];
-                hslot = h.prim;
-                ht = h.t;
+                cont = shade_segment(J, J.rec, hslot, ht, o, d, st, color, atten) && 1u < J.depth;
+                a.col[p] = make_float4(color.x, color.y, color.z, next);
+                if (cont) a.att[p] = make_float4(atten.x, atten.y, atten.z, __uint_as_float(st));
             }
-            cont = shade_segment(J, J.rec, hslot, ht, o, d, st, color, atten) && a.bounce + 1 < J.depth;
-            a.col[p] = make_float4(color.x, color.y, color.z, next);
+            append_block(sh, a, cont, o, d, p);
+        }
+    }
+}
+
+// Shades the queued segments of one bounce (a.bounce) and queues the continuing paths.
+__global__ __launch_bounds__(kShadeBlock) void k_wave_shade(WaveArgs a) {
+    __shared__ BlockAppend sh;
+    const RenderJob &J = a.J;
+    const uint32_t n = *a.n_in;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;  // the same for every thread of the grid
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool cont = false;
+        uint32_t p = 0;
+        v3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f};
+        if (i < n) {
+            p = a.ids_in[i];
+            const float4 *q = reinterpret_cast<const float4 *>(a.rays_in + i);
+            const float4 r0 = q[0], r1 = q[1];
+            o = v3{r0.x, r0.y, r0.z};
+            d = v3{r1.x, r1.y, r1.z};
+            const float4 c = a.col[p], t = a.att[p];
+            v3 color{c.x, c.y, c.z}, atten{t.x, t.y, t.z};
+            uint32_t st = __float_as_uint(t.w);
+            const rt_hit h = a.hits_in[i];
+            cont = shade_segment(J, J.rec, h.prim, h.t, o, d, st, color, atten) && a.bounce + 1 < J.depth;
+            a.col[p] = make_float4(color.x, color.y, color.z, c.w);
             if (cont) a.att[p] = make_float4(atten.x, atten.y, atten.z, __uint_as_float(st));
         }
-        const unsigned long long live = __ballot(cont);
-        if (lane == 0) wave_off[wid] = (uint32_t)__popcll(live);
-        __syncthreads();
-        if (threadIdx.x == 0) {  // exclusive offsets of the block's waves, one global append
-            uint32_t total = 0;
-            for (uint32_t w = 0; w < blockDim.x / 64u; ++w) {
-                const uint32_t c = wave_off[w];
-                wave_off[w] = total;
-                total += c;
-            }
-            block_base = total ? atomicAdd(a.n_out, total) : 0u;
-            if (total && a.segments) atomicAdd(a.segments, (unsigned long long)total);
-        }
-        __syncthreads();
-        if (cont) {
-            const uint32_t slot = block_base + wave_off[wid] + (uint32_t)__popcll(live & lt);
-            store_ray(a.rays_out, slot, o, d);
-            a.ids_out[slot] = p;
-        }
-        __syncthreads();  // wave_off and block_base are rewritten next round
+        append_block(sh, a, cont, o, d, p);
     }
 }
 
@@ -559,6 +588,7 @@ template <int W, int S> __global__ __launch_bounds__(256) void k_wave_finish(Fin
     const uint32_t n = *a.n_in;
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
+    const uint32_t chunk = kRenderChunk;  // queue entries per atomic
     uint32_t phase = kIdle, p = 0, bounce = 0, st = 0, hslot = kEmpty;
     float next = 0.0f, ht = 0.0f;
     v3 color{0.0f, 0.0f, 0.0f}, atten{1.0f, 1.0f, 1.0f};
@@ -575,10 +605,10 @@ template <int W, int S> __global__ __launch_bounds__(256) void k_wave_finish(Fin
             if (q_next == q_end) {
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 uint32_t base = 0;
-                if ((int)lane == leader) base = atomicAdd(f.work, kRenderChunk);
+                if ((int)lane == leader) base = atomicAdd(f.work, chunk);
                 base = (uint32_t)__shfl((int)base, leader);
                 if (base >= n) exhausted = true;
-                else { q_next = base; q_end = min(base + kRenderChunk, n); }
+                else { q_next = base; q_end = min(base + chunk, n); }
             }
             if (!exhausted) {
                 const uint32_t take = min(nidle, q_end - q_next);
@@ -871,7 +901,7 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
         a.rays_out = rays[cur];
         a.ids_out = ids[cur];
         a.n_out = count + cur;
-        k_wave_shade<true><<<grid_for(paths), block, 0, stream>>>(a);
+        k_wave_first<<<std::min(full, grid_for(npix)), block, 0, stream>>>(a);
         RT_HIP(hipGetLastError());
         const uint32_t finish_at = wave_finish_bounce();
         for (uint32_t b = 1; b < j.depth; ++b) {
@@ -902,7 +932,7 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
             a.rays_out = rays[cur ^ 1];
             a.ids_out = ids[cur ^ 1];
             a.n_out = count + (cur ^ 1);
-            k_wave_shade<false><<<std::min(full, grid_for(paths)), block, 0, stream>>>(a);
+            k_wave_shade<<<std::min(full, grid_for(paths)), block, 0, stream>>>(a);
             RT_HIP(hipGetLastError());
             cur ^= 1;
         }

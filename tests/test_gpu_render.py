@@ -224,3 +224,20 @@ def test_default_schedule_of_a_large_image(ctx, native, O):
     assert np.array_equal(gd.view(np.uint32), gp.view(np.uint32))
     assert 0 < segd < segp  # primaries once per pixel instead of once per work item
     ctx.release(sid)
+
+
+@pytest.mark.parametrize("width", [2, 8])
+def test_wavefront_other_bvh_widths_and_an_empty_scene(ctx, native, O, width):
+    """Bounce-by-bounce rendering over BVH4's siblings (W2, W8) equals the oracle; an empty scene
+    renders black without tracing anything."""
+    w, h, spp = 40, 30, 3
+    s = W.icosphere_scene(freq=8, width=w, height=h, spp=spp, depth=5)
+    sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"], width=width)
+    g, _ = gpu_render(native, ctx, sid, s["params"], w, h, spp, native.RENDER_WAVEFRONT)
+    o, _ = osc.render(s["params"], w, h, spp, method=O.WIDE)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    ctx.release(sid)
+    sid = ctx.upload(None, None, s["materials"])
+    g, segs = gpu_render(native, ctx, sid, s["params"], 16, 16, 2, native.RENDER_WAVEFRONT)
+    assert np.all(g[:, :3] == 0.0) and segs == 16 * 16  # primaries traced (and missed), nothing after
+    ctx.release(sid)
