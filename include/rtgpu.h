@@ -205,10 +205,12 @@ typedef struct rt_render_job {
     rt_params params;   /* this image's UBO */
 } rt_render_job;
 
-/* Renders every job's image in one persistent launch per BVH width present (no such
- * call exists in the reference, whose S2 renders one image: RaytracerBVH.cpp:1000-1078).
- * Each image is bit-identical to rt_render of its job alone.  jobs: host array; the
- * accum images must not overlap.  segments (device, may be NULL) as for rt_render. */
+/* Renders every job's image: the ones rt_render would run in one persistent launch share
+ * one launch per BVH width present; the ones it would render bounce by bounce (see
+ * RT_RENDER_WAVEFRONT) follow, each on its own (no such call exists in the reference, whose
+ * S2 renders one image: RaytracerBVH.cpp:1000-1078).  Each image is bit-identical to
+ * rt_render of its job alone.  jobs: host array; the accum images must not overlap.
+ * segments (device, may be NULL) as for rt_render. */
 int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, unsigned long long *segments,
                     const rt_launch_config *cfg, void *stream);
 

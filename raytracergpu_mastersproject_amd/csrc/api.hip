@@ -186,6 +186,7 @@ int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, 
         const LaunchCfg c = resolve_cfg(ctx, ctx->scene(jobs[0].scene_id), cfg, false);
         hipStream_t st = static_cast<hipStream_t>(stream);
         std::map<uint32_t, std::vector<RenderJob>> by_width;
+        std::vector<std::pair<const Scene *, RenderJob>> waves;  // images rendered bounce by bounce, as rt_render would
         std::vector<std::pair<uintptr_t, uintptr_t>> ranges;
         std::vector<std::pair<RenderJob *, uint32_t>> split;  // (job, samples per item)
         uint64_t split_bytes = 0;
@@ -197,7 +198,8 @@ int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, 
             RenderJob rj = make_render_job(s, j.params, j.width, j.height, j.spp, j.flags, j.accum);
             const uintptr_t b = reinterpret_cast<uintptr_t>(j.accum);
             ranges.emplace_back(b, b + 16ull * j.width * j.height);
-            by_width[s.width].push_back(rj);
+            if (j.spp && wave_schedule(j.width, j.height, j.spp, j.flags)) waves.emplace_back(&s, rj);
+            else by_width[s.width].push_back(rj);
         }
         std::sort(ranges.begin(), ranges.end());
         for (size_t i = 1; i < ranges.size(); ++i)
@@ -217,6 +219,7 @@ int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, 
             off += 16ull * rj->width * rj->height * rj->spp;
         }
         for (auto &kv : by_width) launch_render_jobs(ctx, kv.second, kv.first, segments, c, st);
+        for (auto &[s, rj] : waves) launch_render_wave(ctx, *s, rj, segments, c, st, nullptr, 0);
     });
 }
 

@@ -945,15 +945,19 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
     }
 }
 
+bool wave_schedule(uint32_t width, uint32_t height, uint32_t spp, uint32_t flags) {
+    const bool layout = (flags & (RT_RENDER_SPLIT_SAMPLES | RT_RENDER_NO_SPLIT | RT_RENDER_PERSISTENT_PATHS)) != 0;
+    RT_REQUIRE(!(layout && (flags & RT_RENDER_WAVEFRONT)), "RT_RENDER_WAVEFRONT excludes the persistent-launch flags");
+    return (flags & RT_RENDER_WAVEFRONT) || (!layout && (uint64_t)width * height * spp >= (1ull << 22));
+}
+
 void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,
                    uint32_t flags, float *accum, unsigned long long *segments, const LaunchCfg &c,
                    hipStream_t stream) {
     RenderJob j = make_render_job(sc, p, width, height, spp, flags, accum);
     if (spp == 0) return;
     check_stack(sc, c);
-    const bool layout = (flags & (RT_RENDER_SPLIT_SAMPLES | RT_RENDER_NO_SPLIT | RT_RENDER_PERSISTENT_PATHS)) != 0;
-    RT_REQUIRE(!(layout && (flags & RT_RENDER_WAVEFRONT)), "RT_RENDER_WAVEFRONT excludes the persistent-launch flags");
-    if ((flags & RT_RENDER_WAVEFRONT) || (!layout && (uint64_t)width * height * spp >= (1ull << 22))) {
+    if (wave_schedule(width, height, spp, flags)) {
         launch_render_wave(ctx, sc, j, segments, c, stream, nullptr, 0);
         return;
     }

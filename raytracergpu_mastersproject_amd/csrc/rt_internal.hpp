@@ -372,6 +372,8 @@ void launch_accum_clear(float *accum, uint32_t npix, hipStream_t stream);
 // path storage from `scratch` (at least wave_scratch_bytes(width * height)) or, when it is
 // NULL, from the stream-ordered pool.
 uint64_t wave_scratch_bytes(uint64_t paths);
+// rt_render's choice of schedule: bounce by bounce (true) or one persistent launch
+bool wave_schedule(uint32_t width, uint32_t height, uint32_t spp, uint32_t flags);
 uint64_t per_pass_limit(uint64_t npix);  // samples of an npix-pixel image per pass
 void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, unsigned long long *segments,
                         const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes);

@@ -45,8 +45,11 @@ def scenes(ctx, native):
         ctx.release(sid)
 
 
+# (scene, width, height, spp, flags, random state); flags 16 (RT_RENDER_WAVEFRONT) puts a job on the
+# bounce-by-bounce schedule, outside the fused launch
 JOBS = [("soup4", 64, 48, 3, 0, 0), ("ico", 96, 64, 2, 0, 7), ("soup8", 40, 40, 1, 0, 3),
-        ("soup4", 33, 17, 5, 2, 11), ("ico2", 16, 16, 4, 1, 5), ("ico", 8, 8, 9, 0, 1)]
+        ("soup4", 33, 17, 5, 2, 11), ("ico2", 16, 16, 4, 1, 5), ("ico", 8, 8, 9, 0, 1),
+        ("soup8", 24, 20, 3, 16, 9), ("ico", 30, 22, 2, 16, 4)]
 
 
 def test_render_batch_images_equal_single_renders(ctx, native, O, scenes):
