@@ -241,3 +241,34 @@ def test_wavefront_other_bvh_widths_and_an_empty_scene(ctx, native, O, width):
     g, segs = gpu_render(native, ctx, sid, s["params"], 16, 16, 2, native.RENDER_WAVEFRONT)
     assert np.all(g[:, :3] == 0.0) and segs == 16 * 16  # primaries traced (and missed), nothing after
     ctx.release(sid)
+
+
+def test_renders_on_two_streams_share_the_scratch_cache(ctx, native, O):
+    """Bounce-by-bounce renders enqueued on two streams at once take their path storage from the
+    context's scratch cache (a block returns with an event; another stream reuses it only after
+    that event); every image equals the same render done alone."""
+    s = W.icosphere_scene(freq=10, width=96, height=72, spp=6, depth=6)
+    sid = ctx.upload(s["tris"], None, s["materials"])
+    w, h = 96, 72
+    states = [3, 5, 8, 13, 21, 34]
+    alone = []
+    for rs in states:
+        p = s["params"].copy()
+        p["randomState"] = rs
+        g, _ = gpu_render(native, ctx, sid, p, w, h, 6, native.RENDER_WAVEFRONT)
+        alone.append(g)
+    streams = [native.Stream(), native.Stream()]
+    accs = []
+    for k, rs in enumerate(states):  # alternate streams; blocks are reused within a stream at once
+        p = s["params"].copy()
+        p["randomState"] = rs
+        acc = native.DeviceArray(16 * w * h)
+        ctx.accum_clear(acc, w * h, stream=streams[k % 2])
+        ctx.render(sid, p, w, h, 6, acc, flags=native.RENDER_WAVEFRONT, stream=streams[k % 2])
+        accs.append(acc)
+    for st in streams:
+        st.synchronize()
+    for acc, ref in zip(accs, alone):
+        got = acc.to_numpy(np.float32, 4 * w * h).reshape(-1, 4)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    ctx.release(sid)
