@@ -1,35 +1,29 @@
 // render.hip -- path tracing of raytraceBVH.comp (compute stage S2) on the wide BVH.
 //
-// The reference dispatches one sample per pixel per dispatch, spp times, with
-// an image barrier between dispatches (RaytracerBVH.cpp:1025-1050); the
-// samples of one pixel form a chain through the image's alpha channel (the RNG
-// carry).  Pixels are independent, so here one lane runs all spp samples of a
-// pixel in order, keeping the accumulator in registers: the image is read and
-// written once per pixel instead of once per sample, with bit-identical
-// results.  Lanes are persistent and take new pixels (8x8 tiles, so a wave's
-// primary rays are coherent) as soon as they finish; each iteration of the
-// outer loop advances every active lane by one traversal step.
+// The reference dispatches one sample per pixel per dispatch, spp times, with an image
+// barrier between dispatches (RaytracerBVH.cpp:1025-1050); the samples of one pixel form a
+// chain through the image's alpha channel (the RNG carry).  Two schedules reproduce it bit
+// for bit; every path performs the same operations in the same order under both.
 //
-// Primary rays have no jitter (raytraceBVH.comp:338, defocus angle 0), so the
-// first segment of every sample of a pixel is the same ray with the same
-// closest hit; it is traced once per pixel and reused (RT_RENDER_NO_PRIMARY_CACHE
-// disables this).
+// Bounce by bounce (images of 2^22+ pixel-samples, and RT_RENDER_WAVEFRONT): a pass holds
+// whole samples of every pixel.  Primary rays have no jitter (raytraceBVH.comp:338), so one
+// primary ray per pixel is traced by the ray-query kernel; k_wave_first then runs each
+// pixel's samples in order (carry, RNG state, first shading) and queues the paths that
+// scatter; k_wave_finish carries every queued path to its end in one persistent launch; the
+// samples are added to the image in order.  Path state lives in HBM between launches.
 //
-// One launch may render several images (jobs), each with its own scene, camera,
-// image and sample count: the work items of all jobs are concatenated, each job
-// starting on a chunk boundary, so a wave's chunk belongs to one job and is found
-// with one wave-uniform search of the job table.  Since a pixel's samples depend only
-// on its accumulator (the alpha carry), rendering spp = a and then spp = b into the
-// same image equals rendering spp = a + b, which lets a scheduler hand out samples
-// in slices.
-//
-// A small image cannot fill the device one pixel per lane, and a lane running a pixel's
-// samples one after another sets the launch's length.  The carry entering sample s is a
-// pure function of the image alpha, the pixel's seed and s (each sample's first draw is
-// the next carry), so a split job gives every (pixel, sample) its own lane: the lane
-// walks the carry chain to its sample (integer and conversion work only), traces that
-// sample and stores its colour; a second pass adds the samples in order, which keeps the
-// image bit-identical.  Split items trace their own first segment.
+// One persistent launch (smaller images, RT_RENDER_PERSISTENT_PATHS and the layout flags):
+// lanes take work items from 8x8 pixel tiles and run their samples to completion, advancing
+// every active lane by one traversal step per iteration.  A pixel job keeps the accumulator
+// in registers and traces the shared first segment once per pixel.  A split job gives each
+// (pixel, group of samples) its own lane: the carry entering sample s is a pure function of
+// the image alpha, the pixel's seed and s (each sample's first draw is the next carry), so a
+// lane walks the chain to its sample, stores its colours, and a second pass adds the samples
+// in order.  One launch may render several images (jobs), each with its own scene, camera,
+// image and sample count: the work items of all jobs are concatenated, each job starting on a
+// chunk boundary, so a wave's chunk belongs to one job and is found with one wave-uniform
+// search of the job table.  Rendering spp = a and then spp = b into the same image equals
+// rendering spp = a + b, which lets a scheduler hand out samples in slices.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
