@@ -211,19 +211,20 @@ bool enqueue_step(rt_server *s) {
         (first->req.kind == RT_REQ_RENDER ? renders : queries).push_back(first);
     }
     // Renders this step, in admission order: whole while they fit the step's sample budget;
-    // the first that does not fit gets the number of passes that does (at least one when it
-    // is the step's first render).  Every job runs in the split layout with one group size
-    // for the launch, chosen so the launch has ~32 work items per resident lane: its drain
-    // (lanes finishing their last items) stays short and each item traces its first segment
-    // once for its group of samples.  The per-sample storage comes from the scratch half of
-    // the arena and returns there when the step retires.
+    // the first that does not fit gets the number of samples that does (at least one when it
+    // is the step's first render).
     //
     // A render that fills the device by itself (at least ~32 samples per resident lane) runs
-    // alone, as the unbatched path would run it, so the small requests behind it are not
-    // held until the end of a launch that also carries it; small renders are fused.  The
-    // oldest unfinished render decides which kind of step this is.
+    // alone, bounce by bounce as rt_render would run it (below), so the small requests behind
+    // it are not held until the end of a launch that also carries it.  Small renders are fused
+    // into one persistent launch, every job in the split layout with one group size for the
+    // launch, chosen so the launch has ~32 work items per resident lane: its drain (lanes
+    // finishing their last items) stays short and each item traces its first segment once for
+    // its group of samples.  The oldest unfinished render decides which kind of step this is.
     // A step of fused small renders finishes all of them together, so it gets a fifth of the
-    // budget: its first members are not held long by its last.
+    // budget: its first members are not held long by its last.  Storage (path state, or
+    // per-sample colours) comes from the scratch half of the arena and returns there when the
+    // step retires.
     double budget = std::max(1.0, s->sample_rate * s->cfg.step_us);
     std::vector<std::pair<std::shared_ptr<Request>, uint32_t>> plan;
     uint64_t total = 0;
