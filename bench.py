@@ -51,9 +51,10 @@ def load_traffic(name: str):
         return None
 
 
-def cpu_baseline(tris, mats, rays, seconds: float, threads: int):
+def cpu_baseline(tris, mats, rays, seconds: float, threads: int, gpu_hits=None):
     """The oracle (C restatement of the reference's LBVH build + right-first hitBVH traversal),
-    timed on this host on a bounded prefix of the same rays."""
+    timed on this host on a bounded prefix of the same rays.  With the GPU's closest hits of the
+    same rays, also returns the parity of that prefix (see hit_parity)."""
     import oracle as O
     t0 = time.time()
     nodes, _, slots, _, _ = O.build_lbvh(tris)
@@ -61,15 +62,55 @@ def cpu_baseline(tris, mats, rays, seconds: float, threads: int):
     sc = O.OracleScene(tris, None, mats, nodes, slots)
     chunk = 16384
     done, spent = 0, 0.0
+    ref = []
     while done < len(rays) and spent < seconds:
         r = rays[done:done + chunk]
         t0 = time.perf_counter()
-        sc.trace(r, O.REF_BVH, threads=threads)
+        ref.append(sc.trace(r, O.REF_BVH, threads=threads))
         spent += time.perf_counter() - t0
         done += len(r)
-    return {"value": done / spent / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+    base = {"value": done / spent / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"first {done} of {len(rays)} rays (closest-hit), oracle binary-LBVH traversal "
                       f"(RaytracerBVH reference algorithm), {threads} threads; CPU LBVH build {build_s:.1f} s"}
+    parity = None if gpu_hits is None else hit_parity(tris, rays[:done], gpu_hits[:done], np.concatenate(ref))
+    return base, parity
+
+
+def hit_parity(tris, rays, g, o, fp64_sample: int = 1 << 20):
+    """The GPU's closest hits against the oracle's reference traversal on the same rays: primitive
+    ids (mismatches, and the near-ties among them, |dt| < 1e-6 t), bitwise t/u/v where the ids agree;
+    and t, u, v against a float64 Moller-Trumbore intersection of each hit triangle, on up to
+    fp64_sample hits (relative t error, absolute barycentric error; limit 1e-5)."""
+    from raytracergpu_mastersproject_amd import layouts as L
+    hit = o["prim"] != L.NO_HIT
+    mism = g["prim"] != o["prim"]
+    both = hit & (g["prim"] != L.NO_HIT)
+    near = mism & both & (np.abs(g["t"].astype(np.float64) - o["t"]) <= 1e-6 * np.abs(o["t"].astype(np.float64)))
+    same = ~mism & hit
+    bits = same & (g["t"].view(np.uint32) == o["t"].view(np.uint32)) & (g["u"].view(np.uint32) == o["u"].view(np.uint32)) \
+        & (g["v"].view(np.uint32) == o["v"].view(np.uint32))
+    idx = np.nonzero(g["prim"] != L.NO_HIT)[0][:fp64_sample]
+    tr = tris[g["prim"][idx]]
+    v0, v1, v2 = (tr[k][:, :3].astype(np.float64) for k in ("v0", "v1", "v2"))
+    orig, dirn = rays["o"][idx].astype(np.float64), rays["d"][idx].astype(np.float64)
+    e1, e2 = v1 - v0, v2 - v0
+    pv = np.cross(dirn, e2)
+    det = np.einsum("ij,ij->i", e1, pv)
+    sv = orig - v0
+    qv = np.cross(sv, e1)
+    u64 = np.einsum("ij,ij->i", sv, pv) / det
+    w64 = np.einsum("ij,ij->i", dirn, qv) / det
+    t64 = np.einsum("ij,ij->i", e2, qv) / det
+    t_err = np.abs(g["t"][idx] - t64) / np.maximum(np.abs(t64), 1e-30)
+    uv_err = np.maximum(np.abs(g["u"][idx] - u64), np.abs(g["v"][idx] - w64))
+    q = (lambda a, k: float(np.percentile(a, k)) if len(a) else 0.0)
+    return {"rays": int(len(g)), "hits": int(hit.sum()), "prim_mismatch": int(mism.sum()),
+            "near_ties_among_mismatches": int(near.sum()), "bitwise_tuv_where_ids_agree": int(bits.sum()),
+            "fp64_checked": int(len(idx)), "fp64_t_rel_err_max": q(t_err, 100), "fp64_t_rel_err_p99": q(t_err, 99),
+            "fp64_uv_abs_err_max": q(uv_err, 100), "fp64_uv_abs_err_p99": q(uv_err, 99),
+            "fp64_over_1e-5": int(np.sum((t_err > 1e-5) | (uv_err > 1e-5))),
+            "note": "fp64 errors are those of the reference's fp32 plane-based triangleHit, which the GPU's t/u/v "
+                    "reproduce bit for bit (the oracle restates it)"}
 
 
 def copy_calibration(mib: int = 1024, reps: int = 5):
@@ -408,11 +449,14 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
-        cpu = cpu_baseline(tris, mats, rays, args.cpu_seconds, threads)
+        cpu, parity = cpu_baseline(tris, mats, rays, args.cpu_seconds, threads, gpu_hits=hits)
         log(f"cpu baseline: {cpu['value']:.3f} Mrays/s on {threads} threads")
+        log(f"parity vs the reference traversal on {parity['rays']} rays: {parity['prim_mismatch']} id mismatches "
+            f"({parity['near_ties_among_mismatches']} near-ties), fp64 t rel err max {parity['fp64_t_rel_err_max']:.2e}, "
+            f"uv abs err max {parity['fp64_uv_abs_err_max']:.2e}, {parity['fp64_over_1e-5']} over 1e-5")
 
     out = {
         "metric": METRIC,
@@ -445,6 +489,7 @@ def main():
         "kernel_ms_per_launch": round(avg_kernel_ms, 4),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     if fixed is not None:
         out["config"]["launch"] = dict(zip(("block", "lds_stack", "persistent", "refill"),
