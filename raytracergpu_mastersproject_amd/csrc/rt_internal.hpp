@@ -236,8 +236,10 @@ class ScratchCache {
         return best;
     }
 
-    void give(Block *b, hipStream_t st) {
-        RT_HIP(hipEventRecord(b->done, st));
+    // Never throws (a lease gives its block back from a destructor).  A block whose event
+    // cannot be recorded stays taken: its last use could never be known to have completed.
+    void give(Block *b, hipStream_t st) noexcept {
+        if (hipEventRecord(b->done, st) != hipSuccess) return;
         b->last = st;
         b->used = true;
         b->taken = false;
