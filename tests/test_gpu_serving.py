@@ -304,3 +304,22 @@ def test_big_request_runs_bounce_by_bounce(ctx, native, scenes):
         srv.close()
         assert st.status == 0
         assert np.array_equal(out.reshape(-1).view(np.uint32), expect.view(np.uint32))
+
+
+def test_big_and_small_requests_mixed(ctx, native, scenes):
+    """Two device-filling renders (bounce-by-bounce steps) submitted among small renders and ray
+    queries (fused steps): every result equals the request run alone."""
+    ids, params, soup, ico = scenes
+    reqs, expect = _mixed_requests(native, ids, params, _rays(ids, (soup, ico)))
+    for name, w, h, spp, rs in (("ico", 512, 512, 36, 31), ("soup4", 640, 480, 30, 32)):
+        p = with_state(params[name], rs)
+        out = np.zeros((w * h, 4), np.float32)
+        k = len(reqs) // 2  # in the middle of the stream of small requests
+        reqs.insert(k, native.render_request(ids[name], p, w, h, spp, native.REQ_OUT_ACCUM, out=out))
+        expect.append(("render", name, p, w, h, spp, 0, out))
+    srv = native.Server(ctx, arena_bytes=3 << 30, max_active=8, step_us=20000)
+    stats = srv.replay(reqs, np.arange(len(reqs), dtype=np.float64) * 0.5)
+    st = srv.stats()
+    srv.close()
+    assert all(s.status == 0 for s in stats) and st.completed == len(reqs)
+    _check_results(native, ctx, ids, expect)
