@@ -2,7 +2,7 @@
  * rt_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY; never linked by the product).
  *
  * A plain-C restatement of the reference's ray-tracing hot path, used by tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg to check the HIP path's results.
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp).
  *
  * Function -> reference source it restates -> what pins it (tests/test_oracle_*.py):
