@@ -203,16 +203,47 @@ __global__ void k_scan_total(const uint32_t *in, const uint32_t *out, uint32_t n
     if (threadIdx.x == 0) *total = n ? out[n - 1] + in[n - 1] : 0u;
 }
 
-void exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *d_total, hipStream_t st) {
+// Temporaries of one build, carved from one block of the context's scratch cache.  Every
+// launch of a build is on one stream, so a range can be handed out again as soon as the
+// launches using it have been enqueued (mark / release); the block itself returns to the
+// cache with an event after the build's last launch.
+class Temps {
+  public:
+    Temps(rt_context *ctx, uint64_t bytes, hipStream_t st) : lease_(ctx->scratch, bytes, st), cap_(bytes) {}
+    template <class T> T *take(uint64_t count) {
+        const uint64_t b = (sizeof(T) * count + 255u) & ~uint64_t(255);
+        RT_REQUIRE(used_ + b <= cap_, "internal: build temporaries exceed their bound");
+        T *p = reinterpret_cast<T *>(lease_.ptr() + used_);
+        used_ += b;
+        return p;
+    }
+    uint64_t mark() const { return used_; }
+    void release(uint64_t m) { used_ = m; }
+
+  private:
+    ScratchLease lease_;
+    uint64_t cap_, used_ = 0;
+};
+
+uint64_t round256(uint64_t b) { return (b + 255u) & ~uint64_t(255); }
+
+// bytes exclusive_scan takes from Temps for n elements
+uint64_t scan_temp_bytes(uint64_t n) {
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    return nb <= 1 ? 0 : 2 * round256(4 * nb) + scan_temp_bytes(nb);
+}
+
+void exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *d_total, hipStream_t st, Temps &tmp) {
     uint32_t nb = (n + kScanTile - 1) / kScanTile;
     if (nb <= 1) {
         k_scan_block<<<1, kB, 0, st>>>(in, out, n, nullptr);
     } else {
-        DevBuf sums(sizeof(uint32_t) * nb), sums_scan(sizeof(uint32_t) * nb);
-        k_scan_block<<<nb, kB, 0, st>>>(in, out, n, sums.as<uint32_t>());
-        exclusive_scan(sums.as<uint32_t>(), sums_scan.as<uint32_t>(), nb, nullptr, st);
-        k_scan_add<<<nb, kB, 0, st>>>(out, n, sums_scan.as<uint32_t>());
-        RT_HIP(hipStreamSynchronize(st));  // sums buffers are freed on return
+        const uint64_t m = tmp.mark();
+        uint32_t *sums = tmp.take<uint32_t>(nb), *sums_scan = tmp.take<uint32_t>(nb);
+        k_scan_block<<<nb, kB, 0, st>>>(in, out, n, sums);
+        exclusive_scan(sums, sums_scan, nb, nullptr, st, tmp);
+        k_scan_add<<<nb, kB, 0, st>>>(out, n, sums_scan);
+        tmp.release(m);
     }
     if (d_total) k_scan_total<<<1, 64, 0, st>>>(in, out, n, d_total);
 }
@@ -277,21 +308,27 @@ __global__ void k_radix_scatter(const uint32_t *keys_in, const uint32_t *vals_in
     }
 }
 
-void radix_sort(uint32_t *keys, uint32_t *vals, uint32_t n, hipStream_t st) {
+uint64_t radix_temp_bytes(uint64_t n) {
+    const uint64_t nblocks = (n + kSortTile - 1) / kSortTile;
+    return 2 * round256(4 * n) + 2 * round256(4 * 256 * nblocks) + scan_temp_bytes(256 * nblocks);
+}
+
+void radix_sort(uint32_t *keys, uint32_t *vals, uint32_t n, hipStream_t st, Temps &tmp) {
     if (n <= 1) return;
     uint32_t nblocks = (n + kSortTile - 1) / kSortTile;
-    DevBuf k2(sizeof(uint32_t) * n), v2(sizeof(uint32_t) * n);
-    DevBuf hist(sizeof(uint32_t) * 256ull * nblocks), hoff(sizeof(uint32_t) * 256ull * nblocks);
-    uint32_t *ka = keys, *va = vals, *kb = k2.as<uint32_t>(), *vb = v2.as<uint32_t>();
-    for (int pass = 0; pass < 4; ++pass) {
+    const uint64_t m = tmp.mark();
+    uint32_t *k2 = tmp.take<uint32_t>(n), *v2 = tmp.take<uint32_t>(n);
+    uint32_t *hist = tmp.take<uint32_t>(256ull * nblocks), *hoff = tmp.take<uint32_t>(256ull * nblocks);
+    uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
+    for (int pass = 0; pass < 4; ++pass) {  // an even number of passes: the result is back in keys/vals
         uint32_t shift = 8u * pass;
-        k_radix_hist<<<nblocks, kB, 0, st>>>(ka, n, shift, hist.as<uint32_t>(), nblocks);
-        exclusive_scan(hist.as<uint32_t>(), hoff.as<uint32_t>(), 256u * nblocks, nullptr, st);
-        k_radix_scatter<<<nblocks, kB, 0, st>>>(ka, va, kb, vb, n, shift, hoff.as<uint32_t>(), nblocks);
+        k_radix_hist<<<nblocks, kB, 0, st>>>(ka, n, shift, hist, nblocks);
+        exclusive_scan(hist, hoff, 256u * nblocks, nullptr, st, tmp);
+        k_radix_scatter<<<nblocks, kB, 0, st>>>(ka, va, kb, vb, n, shift, hoff, nblocks);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
-    RT_HIP(hipStreamSynchronize(st));  // temporaries are freed on return
+    tmp.release(m);
 }
 
 // ---------------------------------------------------------------------------
@@ -460,31 +497,32 @@ __global__ void k_refit_level(rt_bvh_node *nodes, const uint32_t *depth, uint32_
 }
 
 // Internal boxes of the n-1 internal nodes (leaf boxes are already written).
-void refit_levels(rt_bvh_node *nodes, const uint32_t *parent, uint32_t n, hipStream_t st) {
+uint64_t refit_temp_bytes(uint64_t n) { return 4 * round256(4 * n) + round256(4); }
+
+void refit_levels(rt_bvh_node *nodes, const uint32_t *parent, uint32_t n, hipStream_t st, Temps &tmp) {
     const uint32_t m = n - 1;
-    DevBuf jump(sizeof(uint32_t) * m), dist(sizeof(uint32_t) * m), jump2(sizeof(uint32_t) * m),
-        dist2(sizeof(uint32_t) * m), flag(sizeof(uint32_t));
-    k_depth_init<<<grid_for(m), kB, 0, st>>>(parent, m, jump.as<uint32_t>(), dist.as<uint32_t>());
+    const uint64_t mk = tmp.mark();
+    uint32_t *jump = tmp.take<uint32_t>(m), *dist = tmp.take<uint32_t>(m), *jump2 = tmp.take<uint32_t>(m),
+             *dist2 = tmp.take<uint32_t>(m), *flag = tmp.take<uint32_t>(1);
+    k_depth_init<<<grid_for(m), kB, 0, st>>>(parent, m, jump, dist);
     for (uint32_t round = 0; round < 32; ++round) {  // 2^32 exceeds any depth
-        k_fill_u32<<<1, kB, 0, st>>>(flag.as<uint32_t>(), 0u, 1u);
-        k_depth_jump<<<grid_for(m), kB, 0, st>>>(jump.as<uint32_t>(), dist.as<uint32_t>(), jump2.as<uint32_t>(),
-                                                 dist2.as<uint32_t>(), m, flag.as<uint32_t>());
+        k_fill_u32<<<1, kB, 0, st>>>(flag, 0u, 1u);
+        k_depth_jump<<<grid_for(m), kB, 0, st>>>(jump, dist, jump2, dist2, m, flag);
         std::swap(jump, jump2);
         std::swap(dist, dist2);
         uint32_t more = 0;
-        RT_HIP(hipMemcpyAsync(&more, flag.p, sizeof more, hipMemcpyDeviceToHost, st));
+        RT_HIP(hipMemcpyAsync(&more, flag, sizeof more, hipMemcpyDeviceToHost, st));
         RT_HIP(hipStreamSynchronize(st));
         if (!more) break;
     }
-    k_fill_u32<<<1, kB, 0, st>>>(flag.as<uint32_t>(), 0u, 1u);
-    k_depth_max<<<grid_for(m), kB, 0, st>>>(dist.as<uint32_t>(), m, flag.as<uint32_t>());
+    k_fill_u32<<<1, kB, 0, st>>>(flag, 0u, 1u);
+    k_depth_max<<<grid_for(m), kB, 0, st>>>(dist, m, flag);
     uint32_t deepest = 0;
-    RT_HIP(hipMemcpyAsync(&deepest, flag.p, sizeof deepest, hipMemcpyDeviceToHost, st));
+    RT_HIP(hipMemcpyAsync(&deepest, flag, sizeof deepest, hipMemcpyDeviceToHost, st));
     RT_HIP(hipStreamSynchronize(st));
     RT_REQUIRE(deepest < m, "internal: LBVH parent links do not form a tree");
-    for (uint32_t d = deepest + 1; d-- > 0;)
-        k_refit_level<<<grid_for(m), kB, 0, st>>>(nodes, dist.as<uint32_t>(), m, d);
-    RT_HIP(hipStreamSynchronize(st));  // the temporaries are freed on return
+    for (uint32_t d = deepest + 1; d-- > 0;) k_refit_level<<<grid_for(m), kB, 0, st>>>(nodes, dist, m, d);
+    tmp.release(mk);
 }
 
 // ---------------------------------------------------------------------------
@@ -707,58 +745,72 @@ void build_scene(rt_context *ctx, Scene &sc, const rt_triangle *tris, uint32_t n
     sc.slot_prim.alloc(sizeof(uint32_t) * n);
     sc.slot_mat.alloc(sizeof(uint32_t) * n);
     sc.records.alloc(sizeof(LeafRecord) * n);
-    DevBuf binary(sizeof(rt_bvh_node) * (2ull * n - 1)), range(sizeof(uint2) * std::max<uint32_t>(n - 1, 1));
+    // every temporary of the build from one block of the scratch cache
+    const bool keep_binary = (opts.flags & RT_BUILD_KEEP_BINARY) != 0;
+    const uint32_t max_nodes = std::max<uint32_t>(n - 1, 1);
+    const uint64_t temp_bytes =
+        (keep_binary ? 0 : round256(sizeof(rt_bvh_node) * (2ull * n - 1))) + round256(sizeof(uint2) * max_nodes) +
+        round256(24) + round256(36) + 2 * round256(4ull * n) + round256(4ull * (2ull * n - 1)) +
+        std::max(radix_temp_bytes(n), refit_temp_bytes(n)) +                      // sort, then refit
+        round256(32ull * W * max_nodes) + 4 * round256(4ull * max_nodes) + round256(4) + scan_temp_bytes(max_nodes);
+    Temps tmp(ctx, temp_bytes, st);
+    DevBuf kept;  // the binary nodes, when they stay resident
+    rt_bvh_node *binary = nullptr;
+    if (keep_binary) {
+        kept.alloc(sizeof(rt_bvh_node) * (2ull * n - 1));
+        binary = kept.as<rt_bvh_node>();
+    } else {
+        binary = tmp.take<rt_bvh_node>(2ull * n - 1);
+    }
+    uint2 *range = tmp.take<uint2>(max_nodes);
     if (nodes) {
         std::vector<uint32_t> hslots;
         std::vector<uint2> hrange;
         validate_nodes(nodes, n, ntri, nsph, hslots, hrange, L);
-        RT_HIP(hipMemcpy(binary.p, nodes, sizeof(rt_bvh_node) * (2ull * n - 1), hipMemcpyHostToDevice));
+        RT_HIP(hipMemcpy(binary, nodes, sizeof(rt_bvh_node) * (2ull * n - 1), hipMemcpyHostToDevice));
         RT_HIP(hipMemcpy(sc.slot_prim.p, hslots.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-        RT_HIP(hipMemcpy(range.p, hrange.data(), sizeof(uint2) * hrange.size(), hipMemcpyHostToDevice));
+        RT_HIP(hipMemcpy(range, hrange.data(), sizeof(uint2) * hrange.size(), hipMemcpyHostToDevice));
         k_records<<<grid_for(n), kB, 0, st>>>(dtris.as<rt_triangle>(), dsph.as<rt_sphere>(), sc.slot_prim.as<uint32_t>(),
                                               n, sc.records.as<LeafRecord>(), sc.slot_mat.as<uint32_t>());
     } else {
-        DevBuf bkeys(6 * sizeof(uint32_t)), bounds(9 * sizeof(float));
-        DevBuf keys(sizeof(uint32_t) * n), vals(sizeof(uint32_t) * n), parent(sizeof(uint32_t) * (2ull * n - 1));
-        k_bounds_init<<<1, 64, 0, st>>>(bkeys.as<uint32_t>());
+        const uint64_t mk = tmp.mark();
+        uint32_t *bkeys = tmp.take<uint32_t>(6);
+        float *bounds = tmp.take<float>(9);
+        uint32_t *keys = tmp.take<uint32_t>(n), *vals = tmp.take<uint32_t>(n), *parent = tmp.take<uint32_t>(2ull * n - 1);
+        k_bounds_init<<<1, 64, 0, st>>>(bkeys);
         uint32_t g = std::min<uint32_t>(grid_for(n), 4096);
-        k_centroid_bounds<<<g, kB, 0, st>>>(dtris.as<rt_triangle>(), ntri, dsph.as<rt_sphere>(), n, bkeys.as<uint32_t>());
-        k_bounds_finalize<<<1, 64, 0, st>>>(bkeys.as<uint32_t>(), bounds.as<float>());
-        k_morton<<<grid_for(n), kB, 0, st>>>(dtris.as<rt_triangle>(), ntri, dsph.as<rt_sphere>(), n, bounds.as<float>(),
-                                             (opts.flags & RT_BUILD_REFERENCE_MORTON) ? 1u : 0u, keys.as<uint32_t>(),
-                                             vals.as<uint32_t>());
-        radix_sort(keys.as<uint32_t>(), vals.as<uint32_t>(), n, st);
-        if (n > 1)
-            k_karras<<<grid_for(n - 1), kB, 0, st>>>(keys.as<uint32_t>(), (int)n, binary.as<rt_bvh_node>(),
-                                                     parent.as<uint32_t>(), range.as<uint2>());
-        k_slots<<<grid_for(n), kB, 0, st>>>(vals.as<uint32_t>(), n, ntri, (opts.flags & RT_BUILD_REFERENCE_LEAVES) ? 1u : 0u,
+        k_centroid_bounds<<<g, kB, 0, st>>>(dtris.as<rt_triangle>(), ntri, dsph.as<rt_sphere>(), n, bkeys);
+        k_bounds_finalize<<<1, 64, 0, st>>>(bkeys, bounds);
+        k_morton<<<grid_for(n), kB, 0, st>>>(dtris.as<rt_triangle>(), ntri, dsph.as<rt_sphere>(), n, bounds,
+                                             (opts.flags & RT_BUILD_REFERENCE_MORTON) ? 1u : 0u, keys, vals);
+        radix_sort(keys, vals, n, st, tmp);
+        if (n > 1) k_karras<<<grid_for(n - 1), kB, 0, st>>>(keys, (int)n, binary, parent, range);
+        k_slots<<<grid_for(n), kB, 0, st>>>(vals, n, ntri, (opts.flags & RT_BUILD_REFERENCE_LEAVES) ? 1u : 0u,
                                             sc.slot_prim.as<uint32_t>());
         k_leaves<<<grid_for(n), kB, 0, st>>>(dtris.as<rt_triangle>(), dsph.as<rt_sphere>(), sc.slot_prim.as<uint32_t>(), n,
-                                             binary.as<rt_bvh_node>() + (n - 1), sc.records.as<LeafRecord>(),
-                                             sc.slot_mat.as<uint32_t>());
-        if (n > 1) refit_levels(binary.as<rt_bvh_node>(), parent.as<uint32_t>(), n, st);
-        RT_HIP(hipStreamSynchronize(st));
+                                             binary + (n - 1), sc.records.as<LeafRecord>(), sc.slot_mat.as<uint32_t>());
+        if (n > 1) refit_levels(binary, parent, n, st, tmp);
+        tmp.release(mk);
     }
     RT_HIP(hipGetLastError());
 
     // collapse into the wide layout, BFS order
-    CollapseArgs a{binary.as<rt_bvh_node>(), range.as<uint2>(), n, W, L};
-    const uint32_t max_nodes = std::max<uint32_t>(n - 1, 1);
-    DevBuf wide_tmp((size_t)max_nodes * 32u * W);
-    DevBuf qa(sizeof(uint32_t) * max_nodes), qb(sizeof(uint32_t) * max_nodes);
-    DevBuf newcount(sizeof(uint32_t) * max_nodes), child_base(sizeof(uint32_t) * max_nodes), d_total(sizeof(uint32_t));
-    uint32_t root = 0;
-    RT_HIP(hipMemcpy(qa.p, &root, sizeof(uint32_t), hipMemcpyHostToDevice));
+    CollapseArgs a{binary, range, n, W, L};
+    uint32_t *wide_tmp = tmp.take<uint32_t>(8ull * W * max_nodes);
+    uint32_t *qa = tmp.take<uint32_t>(max_nodes), *qb = tmp.take<uint32_t>(max_nodes);
+    uint32_t *newcount = tmp.take<uint32_t>(max_nodes), *child_base = tmp.take<uint32_t>(max_nodes);
+    uint32_t *d_total = tmp.take<uint32_t>(1);
+    k_fill_u32<<<1, 64, 0, st>>>(qa, 0u, 1u);  // the root
     uint32_t qn = 1, level_base = 0, depth = 0;
-    uint32_t *qcur = qa.as<uint32_t>(), *qnext = qb.as<uint32_t>();
+    uint32_t *qcur = qa, *qnext = qb;
     while (qn > 0) {
-        k_collapse_count<<<grid_for(qn), kB, 0, st>>>(a, qcur, qn, newcount.as<uint32_t>());
-        exclusive_scan(newcount.as<uint32_t>(), child_base.as<uint32_t>(), qn, d_total.as<uint32_t>(), st);
+        k_collapse_count<<<grid_for(qn), kB, 0, st>>>(a, qcur, qn, newcount);
+        exclusive_scan(newcount, child_base, qn, d_total, st, tmp);
         uint32_t next_n = 0;
-        RT_HIP(hipMemcpy(&next_n, d_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        RT_HIP(hipMemcpy(&next_n, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost));
         RT_REQUIRE(level_base + qn + next_n <= max_nodes, "internal: wide node count exceeds bound");
-        k_collapse_emit<<<grid_for(qn), kB, 0, st>>>(a, qcur, qn, level_base, level_base + qn,
-                                                     child_base.as<uint32_t>(), qnext, wide_tmp.as<uint32_t>());
+        k_collapse_emit<<<grid_for(qn), kB, 0, st>>>(a, qcur, qn, level_base, level_base + qn, child_base, qnext,
+                                                     wide_tmp);
         level_base += qn;
         qn = next_n;
         std::swap(qcur, qnext);
@@ -774,12 +826,12 @@ void build_scene(rt_context *ctx, Scene &sc, const rt_triangle *tris, uint32_t n
 
     sc.nwide = level_base;
     sc.wide.alloc((size_t)sc.nwide * 32u * W);
-    RT_HIP(hipMemcpy(sc.wide.p, wide_tmp.p, (size_t)sc.nwide * 32u * W, hipMemcpyDeviceToDevice));
+    RT_HIP(hipMemcpy(sc.wide.p, wide_tmp, (size_t)sc.nwide * 32u * W, hipMemcpyDeviceToDevice));
     sc.wide_depth = depth;
     sc.leaf_size = L;
     sc.stack_bound = (W - 1u) * depth + 1u;
     sc.build_ms = ms;
-    if (opts.flags & RT_BUILD_KEEP_BINARY) sc.binary = std::move(binary);
+    if (keep_binary) sc.binary = std::move(kept);
 }
 
 }  // namespace rtgpu
