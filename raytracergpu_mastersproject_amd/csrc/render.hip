@@ -841,7 +841,7 @@ uint32_t wave_finish_bounce() {
 // Bounce-by-bounce rendering of one image: passes of whole samples of every pixel, each
 // continuing the carry chain of the previous one.  A pass holds at most wave_paths_per_pass()
 // paths, and no more than `scratch` holds when the caller provides it (else the storage comes
-// from the stream-ordered pool).
+// from the context's scratch cache).
 void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, unsigned long long *segments,
                         const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes) {
     const uint32_t npix = j.width * j.height;

@@ -352,8 +352,9 @@ LaunchCfg resolve_cfg(const rt_context *ctx, const Scene &sc, const rt_launch_co
 void check_stack(const Scene &sc, const LaunchCfg &cfg);
 void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t n, rt_hit *hits,
                   uint32_t *counts, uint32_t flags, const LaunchCfg &cfg, hipStream_t stream);
-// Closest-hit query whose hits carry leaf slots (for shading), over rays[0, min(*n_dev, n_max))
-// when n_dev is set (a count produced on the device), else rays[0, n_max); persistent schedule.
+// Closest-hit query whose hits carry leaf slots (for shading).  With n_dev set (a count produced
+// on the device) it covers rays[0, min(*n_dev, n_max)) with the persistent kernel; without, it
+// covers rays[0, n_max) with one thread per ray (camera rays).
 void launch_trace_queue(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t n_max, const uint32_t *n_dev,
                         rt_hit *hits, const LaunchCfg &c, hipStream_t stream);
 void launch_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems, uint32_t flags,
