@@ -174,58 +174,85 @@ __device__ __forceinline__ bool end_sample(const RenderArgs &a, PixelState &ps, 
     return false;
 }
 
-// One resolved segment of a path (raytraceBVH.comp:94-115, 276-308): adds the emitted
-// light and scatters.  Returns true when the path continues, with (o, d) replaced by the
-// next segment's ray; the caller owns the bounce count.  hslot == kEmpty is a miss.
-__device__ __forceinline__ bool shade_segment(const RenderJob &J, const LeafRecord *rec, uint32_t hslot, float ht,
-                                              v3 &o, v3 &d, uint32_t &st, v3 &color, v3 &atten) {
-    if (hslot == kEmpty) {
+// The surface a segment ended on (raytraceBVH.comp:280-300): hit point, face-forward normal
+// and material.  It depends on the segment and its hit only, so the samples of a pixel that
+// share a primary hit share it.
+struct Surface {
+    v3 p, normal, alb;
+    float w;        // albedo.w: metal fuzz, dielectric index of refraction
+    uint32_t type;  // materialType
+    int bf;         // back face
+    bool hit;
+};
+
+__device__ __forceinline__ Surface surface_at(const RenderJob &J, const LeafRecord *rec, uint32_t hslot, float ht,
+                                              v3 o, v3 d) {
+    Surface s{};
+    s.hit = hslot != kEmpty;
+    if (!s.hit) return s;
+    const LeafRecord r = rec[hslot];
+    const rt_material m = J.mats[J.slot_mat[hslot]];
+    s.alb = v3{m.albedo[0], m.albedo[1], m.albedo[2]};
+    s.w = m.albedo[3];
+    s.type = m.materialType;
+    s.p = o + ht * d;
+    v3 n;
+    if (is_sphere_record(r)) n = (s.p - v3{r.r0.x, r.r0.y, r.r0.z}) / r.r0.w;
+    else n = v3{r.r1.w, r.r2.w, r.r3.w};
+    s.bf = dot(d, n) > 0.0f ? 1 : 0;
+    s.normal = (float)(1 - 2 * s.bf) * n;
+    return s;
+}
+
+// Adds the surface's emitted light and scatters the incoming direction d
+// (raytraceBVH.comp:94-115, 276-308).  Returns true when the path continues, with (o, d)
+// replaced by the next segment's ray; the caller owns the bounce count.  A miss adds the
+// background.
+__device__ __forceinline__ bool scatter_at(const RenderJob &J, const Surface &s, v3 &o, v3 &d, uint32_t &st,
+                                           v3 &color, v3 &atten) {
+    if (!s.hit) {
         const v3 background{0.0f, 0.0f, 0.0f};  // _BACKGROUND_COLOR
         color = color + mul(background, atten);
         return false;
     }
-    const LeafRecord r = rec[hslot];
-    const rt_material m = J.mats[J.slot_mat[hslot]];
-    const v3 alb{m.albedo[0], m.albedo[1], m.albedo[2]};
-    const v3 p = o + ht * d;
-    v3 n;
-    if (is_sphere_record(r)) n = (p - v3{r.r0.x, r.r0.y, r.r0.z}) / r.r0.w;
-    else n = v3{r.r1.w, r.r2.w, r.r3.w};
-    const int bf = dot(d, n) > 0.0f ? 1 : 0;
-    const v3 normal = (float)(1 - 2 * bf) * n;
-    const v3 emitted = m.materialType == RT_LIGHT ? alb : v3{0.0f, 0.0f, 0.0f};
+    const v3 emitted = s.type == RT_LIGHT ? s.alb : v3{0.0f, 0.0f, 0.0f};
     color = color + mul(emitted, atten);
     bool scattered = false;
     v3 att{0.0f, 0.0f, 0.0f}, nd{0.0f, 0.0f, 0.0f};
     const bool ref_mats = (J.flags & RT_RENDER_REFERENCE_MATERIALS) != 0;
-    if (m.materialType == RT_DIFFUSE) {
-        att = alb;
+    if (s.type == RT_DIFFUSE) {
+        att = s.alb;
         v3 ruv = normalize(random_in_unit_sphere(st));
-        nd = normalize(normal + ruv);
+        nd = normalize(s.normal + ruv);
         scattered = true;
-    } else if (m.materialType == RT_METALLIC && !ref_mats) {
-        att = alb;
-        v3 refl = reflect(d, normal);
+    } else if (s.type == RT_METALLIC && !ref_mats) {
+        att = s.alb;
+        v3 refl = reflect(d, s.normal);
         v3 rs = random_in_unit_sphere(st);
-        nd = normalize(refl + m.albedo[3] * rs);
-        scattered = dot(nd, normal) > 0.0f;
-    } else if (m.materialType == RT_DIELECTRIC && !ref_mats) {
+        nd = normalize(refl + s.w * rs);
+        scattered = dot(nd, s.normal) > 0.0f;
+    } else if (s.type == RT_DIELECTRIC && !ref_mats) {
         att = v3{1.0f, 1.0f, 1.0f};
-        const float ior = m.albedo[3];
-        const float ri = bf ? ior : (1.0f / ior);
-        const float ct = gmin(dot(-d, normal), 1.0f);
+        const float ri = s.bf ? s.w : (1.0f / s.w);
+        const float ct = gmin(dot(-d, s.normal), 1.0f);
         const float stt = sqrtf(1.0f - ct * ct);
         const bool cannot = ri * stt > 1.0f;
-        if (cannot || schlick(ct, ri) > rng_next(st)) nd = reflect(d, normal);
-        else nd = refract(d, normal, ri);
+        if (cannot || schlick(ct, ri) > rng_next(st)) nd = reflect(d, s.normal);
+        else nd = refract(d, s.normal, ri);
         nd = normalize(nd);
         scattered = true;
     }
     if (!scattered) return false;
     atten = mul(atten, att);
-    o = p;
+    o = s.p;
     d = nd;
     return true;
+}
+
+// One resolved segment of a path: its surface, then the scatter.  hslot == kEmpty is a miss.
+__device__ __forceinline__ bool shade_segment(const RenderJob &J, const LeafRecord *rec, uint32_t hslot, float ht,
+                                              v3 &o, v3 &d, uint32_t &st, v3 &color, v3 &atten) {
+    return scatter_at(J, surface_at(J, rec, hslot, ht, o, d), o, d, st, color, atten);
 }
 
 // a resolved segment of a persistent-kernel path
@@ -500,12 +527,17 @@ __global__ __launch_bounds__(kShadeBlock) void k_wave_first(WaveArgs a) {
         uint32_t x = 0, y = 0, seed0 = 0;
         float carry = 0.0f;
         v3 o0{0.0f, 0.0f, 0.0f}, d0{0.0f, 0.0f, 0.0f};
+        Surface surf{};  // the pixel's primary hit surface, shared by its samples
         if (live) {
             x = pix % J.width;
             y = pix / J.width;
             seed0 = rng_seed(x, y, J.random_state);
             carry = reinterpret_cast<const float4 *>(J.accum)[pix].w;
             primary(J, x, y, o0, d0);
+            if (a.prim_per_pixel && J.depth > 0) {
+                const rt_hit h = a.hits_in[pix];
+                surf = surface_at(J, J.rec, h.prim, h.t, o0, d0);
+            }
         }
         for (uint32_t k = 0; k < a.nsamp; ++k) {
             const uint32_t p = k * a.npix + pix;
@@ -516,14 +548,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_wave_first(WaveArgs a) {
                 const float next = rng_next(st);
                 carry = next;
                 v3 color{0.0f, 0.0f, 0.0f}, atten{1.0f, 1.0f, 1.0f};
-                uint32_t hslot = kEmpty;
-                float ht = 0.0f;
-                if (J.depth > 0) {
-                    const rt_hit h = a.hits_in[a.prim_per_pixel ? pix : p];
-                    hslot = h.prim;
-                    ht = h.t;
+                if (!a.prim_per_pixel && J.depth > 0) {
+                    const rt_hit h = a.hits_in[p];
+                    surf = surface_at(J, J.rec, h.prim, h.t, o, d);
                 }
-                cont = shade_segment(J, J.rec, hslot, ht, o, d, st, color, atten) && 1u < J.depth;
+                cont = scatter_at(J, surf, o, d, st, color, atten) && 1u < J.depth;
                 a.col[p] = make_float4(color.x, color.y, color.z, next);
                 if (cont) a.att[p] = make_float4(atten.x, atten.y, atten.z, __uint_as_float(st));
             }
