@@ -174,6 +174,19 @@ __device__ __forceinline__ bool end_sample(const RenderArgs &a, PixelState &ps, 
     return false;
 }
 
+// Wave-level choice between shading and tracing.  Shading (RNG draws, sampling, scatter, the
+// next segment's setup) is long and only a few lanes finish a segment per iteration, so a wave
+// shades when shade_min of its lanes hold a resolved segment -- or a third of its live lanes,
+// when fewer are live -- or when none is left tracing; the resolved lanes wait meanwhile.
+// Without the second bound a draining wave would shade only once every lane had resolved, and
+// its last paths would advance in lockstep, each bounce waiting for the slowest lane.
+__device__ __forceinline__ bool shade_now(unsigned long long mres, unsigned long long mtr, uint32_t shade_min) {
+    if (mres == 0ull) return false;
+    if (mtr == 0ull) return true;
+    const uint32_t live = (uint32_t)__popcll(mres | mtr);
+    return (uint32_t)__popcll(mres) >= min(shade_min, max(1u, live / 3u));
+}
+
 // The surface a segment ended on (raytraceBVH.comp:280-300): hit point, face-forward normal
 // and material.  It depends on the segment and its hit only, so the samples of a pixel that
 // share a primary hit share it.
@@ -381,12 +394,9 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
             }
         }
         if (__ballot(ps.phase != kIdle) == 0ull) break;
-        // Shading (RNG draws, sampling, scatter, the next segment's setup) is long and only a
-        // few lanes finish a segment per iteration, so a wave shades only when at least
-        // shade_min of its lanes hold a resolved segment, or when none is left tracing; the
-        // resolved lanes wait meanwhile.  A lane's own sequence of segments is unchanged.
+        // A lane's own sequence of segments is the same whichever phase the wave picks.
         const unsigned long long mres = __ballot(ps.phase == kResolved), mtr = __ballot(ps.phase == kTracing);
-        if (mres != 0ull && (mtr == 0ull || (uint32_t)__popcll(mres) >= a.shade_min)) {
+        if (shade_now(mres, mtr, a.shade_min)) {
             // resolved segments may chain (cached primary hit, depth reached): bounded by spp*depth
             while (ps.phase == kResolved) shade<MULTI>(a, ps, t, segs);
         } else {
@@ -655,7 +665,7 @@ template <int W, int S> __global__ __launch_bounds__(256) void k_wave_finish(Fin
         }
         if (__ballot(phase != kIdle) == 0ull) break;
         const unsigned long long mres = __ballot(phase == kResolved), mtr = __ballot(phase == kTracing);
-        if (mres != 0ull && (mtr == 0ull || (uint32_t)__popcll(mres) >= f.shade_min)) {
+        if (shade_now(mres, mtr, f.shade_min)) {
             if (phase == kResolved) {
                 v3 o = t.o, d = t.d;
                 if (shade_segment(J, J.rec, hslot, ht, o, d, st, color, atten) && bounce + 1 < J.depth) {
