@@ -465,6 +465,15 @@ struct WaveArgs {
     rt_ray *rays_out;
     uint32_t *ids_out;
     uint32_t *n_out;
+    // k_wave_first only: when set, paths that scatter off a dielectric go to this queue instead,
+    // and the finish kernel takes them first (rays_in2 ...): they are the ones that enter the
+    // closed meshes and bounce longest, so starting them early keeps them out of the drain.
+    rt_ray *rays_out2;
+    uint32_t *ids_out2;
+    uint32_t *n_out2;
+    const rt_ray *rays_in2;
+    const uint32_t *ids_in2;
+    const uint32_t *n_in2;
     float4 *col;  // [path] {colour, carry out}
     float4 *att;  // [path] {attenuation, RNG state bits}
     unsigned long long *segments;
@@ -491,34 +500,41 @@ __global__ void k_wave_primary(WaveArgs a) {
 constexpr uint32_t kShadeBlock = 1024;
 
 // Appends the block's continuing paths to the queue with one atomic: the block's waves add
-// their counts in LDS first (a counter hit by every wave would serialise them).  Every
-// thread of the block calls it.
+// their counts in LDS first (a counter hit by every wave would serialise them).  Paths with
+// `front` set go to the second queue when the launch has one.  Every thread of the block
+// calls it.
 struct BlockAppend {
-    uint32_t wave_off[kShadeBlock / 64];
-    uint32_t base;
+    uint32_t wave_off[2][kShadeBlock / 64];
+    uint32_t base[2];
 };
 
-__device__ __forceinline__ void append_block(BlockAppend &sh, const WaveArgs &a, bool cont, v3 o, v3 d, uint32_t p) {
+__device__ __forceinline__ void append_block(BlockAppend &sh, const WaveArgs &a, bool cont, bool front, v3 o, v3 d,
+                                             uint32_t p) {
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
-    const unsigned long long live = __ballot(cont);
-    if (lane == 0) sh.wave_off[wid] = (uint32_t)__popcll(live);
+    const uint32_t q = (front && a.n_out2) ? 1u : 0u;
+    const unsigned long long live0 = __ballot(cont && q == 0u), live1 = __ballot(cont && q == 1u);
+    if (lane == 0) {
+        sh.wave_off[0][wid] = (uint32_t)__popcll(live0);
+        sh.wave_off[1][wid] = (uint32_t)__popcll(live1);
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {  // exclusive offsets of the block's waves, one global append
+    if (threadIdx.x < 2u) {  // exclusive offsets of the block's waves, one global append per queue
+        const uint32_t k = threadIdx.x;
         uint32_t total = 0;
         for (uint32_t w = 0; w < blockDim.x / 64u; ++w) {
-            const uint32_t c = sh.wave_off[w];
-            sh.wave_off[w] = total;
+            const uint32_t c = sh.wave_off[k][w];
+            sh.wave_off[k][w] = total;
             total += c;
         }
-        sh.base = total ? atomicAdd(a.n_out, total) : 0u;
+        sh.base[k] = total ? atomicAdd(k ? a.n_out2 : a.n_out, total) : 0u;
         if (total && a.segments) atomicAdd(a.segments, (unsigned long long)total);
     }
     __syncthreads();
     if (cont) {
-        const uint32_t slot = sh.base + sh.wave_off[wid] + (uint32_t)__popcll(live & lt);
-        store_ray(a.rays_out, slot, o, d);
-        a.ids_out[slot] = p;
+        const uint32_t slot = sh.base[q] + sh.wave_off[q][wid] + (uint32_t)__popcll((q ? live1 : live0) & lt);
+        store_ray(q ? a.rays_out2 : a.rays_out, slot, o, d);
+        (q ? a.ids_out2 : a.ids_out)[slot] = p;
     }
     __syncthreads();  // the offsets are rewritten by the next call
 }
@@ -566,7 +582,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wave_first(WaveArgs a) {
                 a.col[p] = make_float4(color.x, color.y, color.z, next);
                 if (cont) a.att[p] = make_float4(atten.x, atten.y, atten.z, __uint_as_float(st));
             }
-            append_block(sh, a, cont, o, d, p);
+            append_block(sh, a, cont, surf.type == RT_DIELECTRIC, o, d, p);
         }
     }
 }
@@ -597,7 +613,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_wave_shade(WaveArgs a) {
             a.col[p] = make_float4(color.x, color.y, color.z, c.w);
             if (cont) a.att[p] = make_float4(atten.x, atten.y, atten.z, __uint_as_float(st));
         }
-        append_block(sh, a, cont, o, d, p);
+        append_block(sh, a, cont, false, o, d, p);
     }
 }
 
@@ -618,7 +634,8 @@ template <int W, int S> __global__ __launch_bounds__(256) void k_wave_finish(Fin
     const StackMem m = stack_mem<S>(lds, spill);
     const WaveArgs &a = f.w;
     const RenderJob &J = a.J;
-    const uint32_t n = *a.n_in;
+    const uint32_t n2 = a.n_in2 ? *a.n_in2 : 0u;  // the second queue's entries come first
+    const uint32_t n = *a.n_in + n2;
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
     const uint32_t chunk = kRenderChunk;  // queue entries per atomic
@@ -648,8 +665,10 @@ template <int W, int S> __global__ __launch_bounds__(256) void k_wave_finish(Fin
                 const uint32_t r = (uint32_t)__popcll(idle & lt);
                 if (phase == kIdle && r < take) {
                     const uint32_t i = q_next + r;
-                    p = a.ids_in[i];
-                    const float4 *q = reinterpret_cast<const float4 *>(a.rays_in + i);
+                    const bool second = i < n2;
+                    const uint32_t qi = second ? i : i - n2;
+                    p = (second ? a.ids_in2 : a.ids_in)[qi];
+                    const float4 *q = reinterpret_cast<const float4 *>((second ? a.rays_in2 : a.rays_in) + qi);
                     const float4 r0 = q[0], r1 = q[1];
                     const float4 c = a.col[p], at = a.att[p];
                     color = v3{c.x, c.y, c.z};
@@ -928,15 +947,27 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
             launch_trace_queue(ctx, sc, rays[0], (uint32_t)nprim, nullptr, hits, tc, stream);
         }
         int cur = 0;
+        const uint32_t finish_at = wave_finish_bounce();
         k_zero<<<1, 64, 0, stream>>>(count + cur);
         a.bounce = 0;
         a.hits_in = hits;
         a.rays_out = rays[cur];
         a.ids_out = ids[cur];
         a.n_out = count + cur;
+        if (finish_at == 1) {  // dielectric scatters into the second queue, which the finish kernel takes first
+            k_zero<<<1, 64, 0, stream>>>(count + 1);
+            a.rays_out2 = rays[1];
+            a.ids_out2 = ids[1];
+            a.n_out2 = count + 1;
+        }
         k_wave_first<<<std::min(full, grid_for(npix)), block, 0, stream>>>(a);
         RT_HIP(hipGetLastError());
-        const uint32_t finish_at = wave_finish_bounce();
+        a.rays_in2 = a.rays_out2;
+        a.ids_in2 = a.ids_out2;
+        a.n_in2 = a.n_out2;
+        a.rays_out2 = nullptr;
+        a.ids_out2 = nullptr;
+        a.n_out2 = nullptr;
         for (uint32_t b = 1; b < j.depth; ++b) {
             if (b >= finish_at) {  // the remaining bounces of the queued paths in one launch
                 a.bounce = b;
