@@ -499,24 +499,36 @@ __global__ void k_wave_primary(WaveArgs a) {
 
 constexpr uint32_t kShadeBlock = 1024;
 
-// Appends the block's continuing paths to the queue with one atomic: the block's waves add
-// their counts in LDS first (a counter hit by every wave would serialise them).  Paths with
-// `front` set go to the second queue when the launch has one.  Every thread of the block
-// calls it.
+// Appends the block's continuing paths to the queue with one atomic per queue: the block's
+// waves add their counts in LDS first (a counter hit by every wave would serialise them).
+// Each thread offers G paths at once (p0, p0 + pstride, ...), so a pass's samples take G times
+// fewer block-wide rounds.  Paths with `front` set go to the second queue when the launch has
+// one.  Every thread of the block calls it.
 struct BlockAppend {
     uint32_t wave_off[2][kShadeBlock / 64];
     uint32_t base[2];
 };
 
-__device__ __forceinline__ void append_block(BlockAppend &sh, const WaveArgs &a, bool cont, bool front, v3 o, v3 d,
-                                             uint32_t p) {
+template <int G>
+__device__ __forceinline__ void append_block(BlockAppend &sh, const WaveArgs &a, const bool (&cont)[G],
+                                             const bool (&front)[G], const v3 (&o)[G], const v3 (&d)[G],
+                                             uint32_t p0, uint32_t pstride) {
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
-    const uint32_t q = (front && a.n_out2) ? 1u : 0u;
-    const unsigned long long live0 = __ballot(cont && q == 0u), live1 = __ballot(cont && q == 1u);
+    const bool two = a.n_out2 != nullptr;
+    unsigned long long live[2][G];
+    uint32_t cnt[2] = {0u, 0u};
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const bool q = front[j] && two;
+        live[0][j] = __ballot(cont[j] && !q);
+        live[1][j] = __ballot(cont[j] && q);
+        cnt[0] += (uint32_t)__popcll(live[0][j]);
+        cnt[1] += (uint32_t)__popcll(live[1][j]);
+    }
     if (lane == 0) {
-        sh.wave_off[0][wid] = (uint32_t)__popcll(live0);
-        sh.wave_off[1][wid] = (uint32_t)__popcll(live1);
+        sh.wave_off[0][wid] = cnt[0];
+        sh.wave_off[1][wid] = cnt[1];
     }
     __syncthreads();
     if (threadIdx.x < 2u) {  // exclusive offsets of the block's waves, one global append per queue
@@ -531,13 +543,23 @@ __device__ __forceinline__ void append_block(BlockAppend &sh, const WaveArgs &a,
         if (total && a.segments) atomicAdd(a.segments, (unsigned long long)total);
     }
     __syncthreads();
-    if (cont) {
-        const uint32_t slot = sh.base[q] + sh.wave_off[q][wid] + (uint32_t)__popcll((q ? live1 : live0) & lt);
-        store_ray(q ? a.rays_out2 : a.rays_out, slot, o, d);
-        (q ? a.ids_out2 : a.ids_out)[slot] = p;
+    uint32_t before[2] = {0u, 0u};  // this wave's paths of the earlier samples of the group
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const uint32_t q = (front[j] && two) ? 1u : 0u;
+        if (cont[j]) {
+            const uint32_t slot =
+                sh.base[q] + sh.wave_off[q][wid] + before[q] + (uint32_t)__popcll(live[q][j] & lt);
+            store_ray(q ? a.rays_out2 : a.rays_out, slot, o[j], d[j]);
+            (q ? a.ids_out2 : a.ids_out)[slot] = p0 + (uint32_t)j * pstride;
+        }
+        before[0] += (uint32_t)__popcll(live[0][j]);
+        before[1] += (uint32_t)__popcll(live[1][j]);
     }
     __syncthreads();  // the offsets are rewritten by the next call
 }
+
+constexpr int kFirstGroup = 4;  // samples a bounce-0 thread appends at once
 
 // Bounce 0 of a pass, one thread per pixel: its samples in order, each starting from the carry
 // the previous one left (raytraceBVH.comp:353-375), from the pixel's primary ray and its
@@ -565,24 +587,32 @@ __global__ __launch_bounds__(kShadeBlock) void k_wave_first(WaveArgs a) {
                 surf = surface_at(J, J.rec, h.prim, h.t, o0, d0);
             }
         }
-        for (uint32_t k = 0; k < a.nsamp; ++k) {
-            const uint32_t p = k * a.npix + pix;
-            bool cont = false;
-            v3 o = o0, d = d0;
-            if (live) {
-                uint32_t st = seed0 + f2u_sat(carry * 4294967294.0f);  // the literal is 2^32 in fp32
-                const float next = rng_next(st);
-                carry = next;
-                v3 color{0.0f, 0.0f, 0.0f}, atten{1.0f, 1.0f, 1.0f};
-                if (!a.prim_per_pixel && J.depth > 0) {
-                    const rt_hit h = a.hits_in[p];
-                    surf = surface_at(J, J.rec, h.prim, h.t, o, d);
+        for (uint32_t k0 = 0; k0 < a.nsamp; k0 += kFirstGroup) {
+            bool cont[kFirstGroup], front[kFirstGroup];
+            v3 o[kFirstGroup], d[kFirstGroup];
+#pragma unroll
+            for (int j = 0; j < kFirstGroup; ++j) {
+                const uint32_t k = k0 + (uint32_t)j;
+                const uint32_t p = k * a.npix + pix;
+                cont[j] = false;
+                o[j] = o0;
+                d[j] = d0;
+                if (live && k < a.nsamp) {
+                    uint32_t st = seed0 + f2u_sat(carry * 4294967294.0f);  // the literal is 2^32 in fp32
+                    const float next = rng_next(st);
+                    carry = next;
+                    v3 color{0.0f, 0.0f, 0.0f}, atten{1.0f, 1.0f, 1.0f};
+                    if (!a.prim_per_pixel && J.depth > 0) {
+                        const rt_hit h = a.hits_in[p];
+                        surf = surface_at(J, J.rec, h.prim, h.t, o[j], d[j]);
+                    }
+                    cont[j] = scatter_at(J, surf, o[j], d[j], st, color, atten) && 1u < J.depth;
+                    a.col[p] = make_float4(color.x, color.y, color.z, next);
+                    if (cont[j]) a.att[p] = make_float4(atten.x, atten.y, atten.z, __uint_as_float(st));
                 }
-                cont = scatter_at(J, surf, o, d, st, color, atten) && 1u < J.depth;
-                a.col[p] = make_float4(color.x, color.y, color.z, next);
-                if (cont) a.att[p] = make_float4(atten.x, atten.y, atten.z, __uint_as_float(st));
+                front[j] = surf.type == RT_DIELECTRIC;
             }
-            append_block(sh, a, cont, surf.type == RT_DIELECTRIC, o, d, p);
+            append_block<kFirstGroup>(sh, a, cont, front, o, d, k0 * a.npix + pix, a.npix);
         }
     }
 }
@@ -613,7 +643,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_wave_shade(WaveArgs a) {
             a.col[p] = make_float4(color.x, color.y, color.z, c.w);
             if (cont) a.att[p] = make_float4(atten.x, atten.y, atten.z, __uint_as_float(st));
         }
-        append_block(sh, a, cont, false, o, d, p);
+        const bool c1[1] = {cont}, f1[1] = {false};
+        const v3 o1[1] = {o}, d1[1] = {d};
+        append_block<1>(sh, a, c1, f1, o1, d1, p, 0u);
     }
 }
 
