@@ -497,7 +497,7 @@ __global__ void k_wave_primary(WaveArgs a) {
     if (a.segments && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.segments, (unsigned long long)n);
 }
 
-constexpr uint32_t kShadeBlock = 1024;
+constexpr uint32_t kShadeBlock = 512;
 
 // Appends the block's continuing paths to the queue with one atomic per queue: the block's
 // waves add their counts in LDS first (a counter hit by every wave would serialise them).
