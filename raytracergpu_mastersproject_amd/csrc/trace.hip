@@ -293,7 +293,7 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
     const size_t lds = stack_lds_bytes(c.lds_stack, c.block);
     if (!c.persistent) {
         void *k = pick_kernel(sc.width, c.lds_stack, 0, any, counts != nullptr);
-        uint32_t grid = (n + c.block - 1) / c.block;
+        const uint32_t grid = (uint32_t)(((uint64_t)n + c.block - 1) / c.block);
         void *args[] = {&a};
         RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
     } else {
@@ -322,7 +322,7 @@ void launch_trace_queue(rt_context *ctx, const Scene &sc, const rt_ray *rays, ui
         // refilling lanes in a persistent wave mixes rays at different depths instead.
         void *k = pick_kernel(sc.width, c.lds_stack, 0, false, false);
         void *args[] = {&a};
-        RT_HIP(hipLaunchKernel(k, dim3((n_max + c.block - 1) / c.block), dim3(c.block), args, lds, stream));
+        RT_HIP(hipLaunchKernel(k, dim3((uint32_t)(((uint64_t)n_max + c.block - 1) / c.block)), dim3(c.block), args, lds, stream));
         RT_HIP(hipGetLastError());
         return;
     }
