@@ -83,10 +83,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
     bool exhausted = false;
     uint32_t q_next = 0, q_end = 0;  // the wave's current chunk of ray indices (wave-uniform)
     const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;
-    // rays per atomic: kChunk, or fewer when there are too few rays for every wave to take two
-    // chunks (a small launch then ends with its longest ray, not with a wave's run of rays)
-    const uint32_t waves = gridDim.x * (blockDim.x / 64u);
-    const uint32_t chunk = max(64u, min(kChunk, (n / (2u * waves)) & ~63u));
+    const ChunkSchedule<uint32_t> sched(n, gridDim.x * (blockDim.x / 64u), true);
     Trav t;
     t.have = false;
     t.top = 0;
@@ -97,10 +94,11 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
             if (q_next == q_end) {  // one atomic per chunk keeps the shared counter off the critical path
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 uint32_t base = 0;
-                if ((int)lane == leader) base = atomicAdd(a.work, chunk);
+                if ((int)lane == leader) base = atomicAdd(a.work, 1u);
                 base = (uint32_t)__shfl((int)base, leader);
-                if (base >= n) exhausted = true;
-                else { q_next = base; q_end = (uint32_t)min((uint64_t)base + chunk, (uint64_t)n); }
+                uint64_t lo, hi;
+                if (!sched.range(base, lo, hi)) exhausted = true;
+                else { q_next = (uint32_t)lo; q_end = (uint32_t)hi; }
             }
             if (!exhausted) {
                 const uint32_t take = min(nidle, q_end - q_next);
@@ -170,6 +168,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
     bool active = false, exhausted = false;
     uint32_t item = 0, local = 0;
     uint64_t q_next = 0, q_end = 0;  // the wave's current chunk of global ray indices
+    const ChunkSchedule<uint64_t> sched(a.total, gridDim.x * (blockDim.x / 64u), false);
     Trav t;
     t.have = false;
     t.top = 0;
@@ -180,10 +179,9 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
             if (q_next == q_end) {
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 unsigned long long base = 0;
-                if ((int)lane == leader) base = atomicAdd(a.work, (unsigned long long)kChunk);
+                if ((int)lane == leader) base = atomicAdd(a.work, 1ull);
                 base = __shfl(base, leader);
-                if (base >= a.total) exhausted = true;
-                else { q_next = base; q_end = min((uint64_t)base + kChunk, a.total); }
+                if (!sched.range(base, q_next, q_end)) exhausted = true;
             }
             const uint32_t take = exhausted ? 0u : (uint32_t)min((uint64_t)nidle, q_end - q_next);
             const uint32_t r = (uint32_t)__popcll(idle & lt);

@@ -29,6 +29,45 @@ namespace rtgpu {
 constexpr uint32_t kSpillCap = 184;  // private spill entries per lane (beyond the LDS part)
 constexpr uint32_t kChunk = 512;     // work items a wave takes from the global counter per atomic
 
+// Which rays a wave takes per atomic.  The counter counts chunks, not rays: the first rays go in
+// chunks of `chunk`, the last 10 per resident lane in chunks of 128 and then 64.  When the
+// counter runs out, a wave is then left with at most one or two rays per lane instead of a
+// whole chunk (8 rays per lane at 512), and the launch drains sooner; the few thousand extra
+// atomics cost less than that drain.
+// T is the ray-index type: uint32_t for one batch of rays, uint64_t for the fused many-scene
+// launch.
+template <typename T> struct ChunkSchedule {
+    T n, nA, nB, KA, KB;
+    uint32_t chunk;
+
+    // shrink: fewer rays per chunk when there are too few for every wave to take two chunks
+    // (a small launch then ends with its longest ray, not with a wave's run of rays)
+    __device__ __forceinline__ ChunkSchedule(T n_, uint32_t waves, bool shrink) {
+        n = n_;
+        chunk = shrink ? (uint32_t)max((T)64u, min((T)kChunk, (T)(n / (2u * waves)) & ~(T)63u)) : kChunk;
+        const T lanes = (T)waves * 64u;
+        const T nC = chunk > 64u ? min(n, (T)(2u * lanes)) : (T)0u;
+        nB = chunk > 128u ? min((T)(n - nC), (T)(8u * lanes)) : (T)0u;
+        nA = n - nB - nC;
+        KA = (nA + chunk - 1u) / chunk;
+        KB = (nB + 127u) / 128u;
+    }
+    // rays [lo, hi) of chunk c; false when c is past the end
+    __device__ __forceinline__ bool range(T c, uint64_t &lo, uint64_t &hi) const {
+        if (c < KA) {
+            lo = (uint64_t)c * chunk;
+            hi = min(lo + chunk, (uint64_t)nA);
+        } else if (c - KA < KB) {
+            lo = (uint64_t)nA + (uint64_t)(c - KA) * 128u;
+            hi = min(lo + 128u, (uint64_t)nA + nB);
+        } else {
+            lo = (uint64_t)nA + nB + (uint64_t)(c - KA - KB) * 64u;
+            hi = min(lo + 64u, (uint64_t)n);
+        }
+        return lo < (uint64_t)n;
+    }
+};
+
 template <int W> struct WideNode {
     float lo[3][W], hi[3][W];
     uint32_t child[W];
