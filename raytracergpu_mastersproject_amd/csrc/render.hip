@@ -7,10 +7,12 @@
 //
 // Bounce by bounce (images of 2^22+ pixel-samples, and RT_RENDER_WAVEFRONT): a pass holds
 // whole samples of every pixel.  Primary rays have no jitter (raytraceBVH.comp:338), so one
-// primary ray per pixel is traced by the ray-query kernel; k_wave_first then runs each
-// pixel's samples in order (carry, RNG state, first shading) and queues the paths that
-// scatter; k_wave_finish carries every queued path to its end in one persistent launch; the
-// samples are added to the image in order.  Path state lives in HBM between launches.
+// primary ray per pixel is traced, one thread per ray; k_wave_first then runs each pixel's
+// samples in order (carry, RNG state, first shading off the pixel's primary surface) and
+// queues the paths that scatter, those off dielectrics in a queue of their own;
+// k_wave_finish carries every queued path to its end in one persistent launch, the
+// dielectric queue first (those paths enter closed meshes and bounce longest); the samples
+// are added to the image in order.  Path state lives in HBM between launches.
 //
 // One persistent launch (smaller images, RT_RENDER_PERSISTENT_PATHS and the layout flags):
 // lanes take work items from 8x8 pixel tiles and run their samples to completion, advancing
