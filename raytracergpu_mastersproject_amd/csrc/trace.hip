@@ -148,8 +148,8 @@ struct BatchArgs {
     uint32_t leaf_weight;
 };
 
-__device__ __forceinline__ uint32_t find_item(const BatchItemDev *items, uint32_t nitems, uint64_t g) {
-    uint32_t lo = 0, hi = nitems;  // last item with first <= g
+// the last item in [lo, hi) whose first ray is at or before g
+__device__ __forceinline__ uint32_t find_item(const BatchItemDev *items, uint32_t lo, uint32_t hi, uint64_t g) {
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
         if (items[mid].first <= g) lo = mid;
@@ -168,6 +168,11 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
     bool active = false, exhausted = false;
     uint32_t item = 0, local = 0;
     uint64_t q_next = 0, q_end = 0;  // the wave's current chunk of global ray indices
+    uint32_t c_lo = 0, c_hi = 1;     // the items the chunk's rays belong to: [c_lo, c_hi)
+    // the lane's scene, kept in registers: a table load per traversal step would add its
+    // latency to every step
+    const uint32_t *wide = nullptr;
+    const LeafRecord *rec = nullptr;
     const ChunkSchedule<uint64_t> sched(a.total, gridDim.x * (blockDim.x / 64u), false);
     Trav t;
     t.have = false;
@@ -181,15 +186,22 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
                 unsigned long long base = 0;
                 if ((int)lane == leader) base = atomicAdd(a.work, 1ull);
                 base = __shfl(base, leader);
-                if (!sched.range(base, q_next, q_end)) exhausted = true;
+                if (!sched.range(base, q_next, q_end)) {
+                    exhausted = true;
+                } else {  // chunks are short: usually one item, found once per chunk
+                    c_lo = find_item(a.items, 0u, a.nitems, q_next);
+                    c_hi = find_item(a.items, c_lo, a.nitems, q_end - 1u) + 1u;
+                }
             }
             const uint32_t take = exhausted ? 0u : (uint32_t)min((uint64_t)nidle, q_end - q_next);
             const uint32_t r = (uint32_t)__popcll(idle & lt);
             const uint64_t g = q_next + r;
             q_next += take;
             if (!active && r < take) {
-                item = find_item(a.items, a.nitems, g);
+                item = find_item(a.items, c_lo, c_hi, g);
                 local = (uint32_t)(g - a.items[item].first);
+                wide = a.items[item].sc.wide;
+                rec = a.items[item].sc.rec;
                 v3 o, d;
                 float tmin, tmax;
                 load_ray(a.items[item].rays, local, o, tmin, d, tmax);
@@ -200,9 +212,9 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
         if (__ballot(active) == 0ull) break;
         const bool on_node = active && at_node(t), on_leaf = active && at_leaf(t);
         if (choose_leaf_phase(on_node, on_leaf, a.leaf_weight)) {
-            if (on_leaf) leaf_step<S, ANY, false>(t, m, a.items[item].sc.rec);
+            if (on_leaf) leaf_step<S, ANY, false>(t, m, rec);
         } else {
-            if (on_node) node_step<W, S, ANY, false>(t, m, a.items[item].sc.wide);
+            if (on_node) node_step<W, S, ANY, false>(t, m, wide);
         }
         if (active && trav_done(t)) {
             store_hit(a.items[item].sc, t, a.items[item].hits, nullptr, local);
