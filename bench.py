@@ -8,7 +8,8 @@ One step = one rt_trace launch over all rays.
 Also reported: any-hit rate on the same rays, the algorithmic-bytes roofline of the dominant
 kernel, and a CPU baseline (the oracle's restatement of the reference's own binary-LBVH
 traversal, multithreaded, on a bounded ray sample); then, as sub-objects of the same JSON line,
-path tracing of the 1M-triangle icosphere at 1080p/16 spp ("pathtrace"), the 256-scene fused
+path tracing of the 1M-triangle icosphere at 1080p/16 spp ("pathtrace") and of the reference's own
+1,028-triangle soup at 256x256/1 spp ("soup", with its whole-frame oracle time), the 256-scene fused
 launch with its build and launch tuners ("batched"), and the serving loop's p50/p99 latency at
 Poisson loads of 0.5x and 0.8x the unbatched capacity ("serving").
 
@@ -190,6 +191,57 @@ def pathtrace_bench(ctx, N, W, L, frames: int, cpu_seconds: float, threads: int)
         out["cpu_baseline"] = {"value": round(segc / spent / 1e6, 3), "unit": "Msegments/s", "cores": threads,
                                "kind": "port", "sample": f"{rows} image rows ({rows * w} pixels x {spp} spp) of the "
                                f"same frame from row {h // 2}, oracle path tracer on its reference LBVH traversal"}
+    ctx.release(sid)
+    return out
+
+
+def soup_bench(ctx, N, W, frames: int, cpu: bool, threads: int):
+    """The reference's own small case: 1,024 random triangles + ground + emissive quad (1,028 triangles),
+    256x256, 1 spp, depth 8, randomState 0.  One frame = rt_accum_clear + rt_render on the resident BVH;
+    the scene upload with its device build is timed apart (the reference rebuilds every frame).  The image
+    is compared bit for bit with the oracle's, which also gives the CPU time of the whole frame."""
+    s = W.soup_scene()
+    w, h, spp = s["width"], s["height"], s["spp"]
+    npix = w * h
+    N.synchronize()
+    t0 = time.perf_counter()
+    sid = ctx.upload(s["tris"], None, s["materials"])
+    N.synchronize()
+    upload_ms = (time.perf_counter() - t0) * 1e3
+    acc = N.DeviceArray(16 * npix)
+    stream = N.Stream()
+
+    def frame():
+        ctx.accum_clear(acc, npix, stream=stream)
+        ctx.render(sid, s["params"], w, h, spp, acc, stream=stream)
+
+    for _ in range(3):
+        frame()
+    ev = [N.Event() for _ in range(frames + 1)]
+    ev[0].record(stream)
+    for k in range(frames):
+        frame()
+        ev[k + 1].record(stream)
+    stream.synchronize()
+    per = sorted(ev[k].elapsed_ms(ev[k + 1]) for k in range(frames))
+    ms = per[len(per) // 2]
+    img = acc.to_numpy(np.float32, 4 * npix).reshape(npix, 4)
+    out = {"workload": f"soup: {len(s['tris'])} triangles (1,024 random + ground + emissive quad), {w}x{h}, {spp} spp, "
+                       "depth 8, randomState 0",
+           "ms_per_frame_median": round(ms, 4), "msamples_per_s": round(npix * spp / (ms * 1e-3) / 1e6, 1),
+           "scene_upload_and_build_ms": round(upload_ms, 2)}
+    if cpu:
+        import oracle as O
+        nodes, _, slots, _, _ = O.build_lbvh(s["tris"])
+        osc = O.OracleScene(s["tris"], None, s["materials"], nodes, slots)
+        t1 = time.perf_counter()
+        o, _ = osc.render(s["params"], w, h, spp, method=O.REF_BVH, threads=threads)
+        cpu_ms = (time.perf_counter() - t1) * 1e3
+        out["image_bit_exact_vs_oracle"] = bool(np.array_equal(img.view(np.uint32), o.view(np.uint32)))
+        out["cpu_baseline"] = {"value": round(cpu_ms, 2), "unit": "ms per frame", "cores": threads, "kind": "port",
+                               "sample": "the whole frame, oracle path tracer on its reference LBVH traversal"}
+    log(f"soup: {ms:.3f} ms/frame (median of {frames}), upload+build {upload_ms:.1f} ms"
+        + (f", oracle {out['cpu_baseline']['value']} ms, bit-exact {out['image_bit_exact_vs_oracle']}" if cpu else ""))
     ctx.release(sid)
     return out
 
@@ -500,6 +552,9 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
         out["pathtrace"] = pathtrace_bench(ctx, N, W, L, args.pt_frames,
                                            0.0 if args.no_cpu_baseline else args.cpu_seconds / 2, threads)
+    if not args.no_pathtrace:
+        threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
+        out["soup"] = soup_bench(ctx, N, W, 50, not args.no_cpu_baseline, threads)
     build = N.BuildOptions(0, inf.width, inf.leaf_size, 0)
     if not args.no_batched:
         out["batched"], build = batched_bench(ctx, N, W, L, args.batched_scenes, 5)
