@@ -235,17 +235,18 @@ __device__ __forceinline__ bool scatter_at(const RenderJob &J, const Surface &s,
     bool scattered = false;
     v3 att{0.0f, 0.0f, 0.0f}, nd{0.0f, 0.0f, 0.0f};
     const bool ref_mats = (J.flags & RT_RENDER_REFERENCE_MATERIALS) != 0;
-    if (s.type == RT_DIFFUSE) {
+    if (s.type == RT_DIFFUSE || (s.type == RT_METALLIC && !ref_mats)) {
+        // both draw one point in the unit sphere: one pass of the sampling code for a wave
+        // that holds both kinds of lanes
         att = s.alb;
-        v3 ruv = normalize(random_in_unit_sphere(st));
-        nd = normalize(s.normal + ruv);
-        scattered = true;
-    } else if (s.type == RT_METALLIC && !ref_mats) {
-        att = s.alb;
-        v3 refl = reflect(d, s.normal);
-        v3 rs = random_in_unit_sphere(st);
-        nd = normalize(refl + s.w * rs);
-        scattered = dot(nd, s.normal) > 0.0f;
+        const v3 rs = random_in_unit_sphere(st);
+        if (s.type == RT_DIFFUSE) {
+            nd = normalize(s.normal + normalize(rs));
+            scattered = true;
+        } else {
+            nd = normalize(reflect(d, s.normal) + s.w * rs);
+            scattered = dot(nd, s.normal) > 0.0f;
+        }
     } else if (s.type == RT_DIELECTRIC && !ref_mats) {
         att = v3{1.0f, 1.0f, 1.0f};
         const float ri = s.bf ? s.w : (1.0f / s.w);
