@@ -17,6 +17,7 @@ Fused many-scene ray query: 256 scenes (65.6 M triangles), 55.1 M rays in batche
 
 Path tracing: the 1,003,522-triangle icosphere, 1920x1080, 16 spp, depth 8 (rendered bounce by
 bounce by default at this size).
+- The device builder's binary LBVH of the icosphere equals the oracle's builder byte for byte.
 - Bands of rows (the middle of the sphere, and rows from 600 where the reference's seed
   formula aliases) equal the oracle's path tracer bit for bit, RGBA including the RNG carry.
 - The whole image equals the single persistent launch's image: the schedule changes when a
@@ -111,6 +112,9 @@ def test_path_traced_frame_at_full_size(ctx, native, O):
     assert (w, h, spp, int(s["params"]["maxRayTraceDepth"])) == (1920, 1080, 16, 8)
     print(f"full-size frame: {len(s['tris'])} triangles, {w}x{h}, {spp} spp", flush=True)
     sid, osc = upload_both(ctx, O, s["tris"], None, s["materials"])
+    nodes, _, slots, _, _ = O.build_lbvh(s["tris"])
+    assert np.array_equal(ctx.nodes(sid).view(np.uint8), np.ascontiguousarray(nodes).view(np.uint8))
+    assert np.array_equal(ctx.slots(sid), slots)
     g = _render(native, ctx, sid, s)
     assert np.all(np.isfinite(g)) and np.all(g[..., :3] >= 0.0)
     lit = g[..., :3].sum(axis=2) > 0.0
