@@ -172,11 +172,11 @@ int rt_trace(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n,
 #define RT_RENDER_SPLIT_SAMPLES 4u       /* always one lane per (pixel, sample) */
 #define RT_RENDER_NO_SPLIT 8u            /* always one lane per pixel */
 /* How paths are scheduled.  An image of at least 2^22 pixel-samples is rendered bounce by
- * bounce by default: the ray-query kernel traces one segment of every live path from a
- * queue of rays, then a shading kernel writes the next bounce's queue (primary segments are
- * traced once per pixel).  Smaller images carry every path to its end inside one persistent
- * launch.  The image is the same either way.  RT_RENDER_SPLIT_SAMPLES and RT_RENDER_NO_SPLIT
- * describe the persistent launch and select it. */
+ * bounce by default: the primary segments are traced once per pixel, a shading kernel runs
+ * every sample's first bounce and queues the paths that scatter, and one persistent launch
+ * carries every queued path to its end.  Smaller images carry every path to its end inside
+ * one persistent launch from the start.  The image is the same either way.
+ * RT_RENDER_SPLIT_SAMPLES and RT_RENDER_NO_SPLIT describe the persistent launch and select it. */
 #define RT_RENDER_WAVEFRONT 16u          /* always bounce by bounce */
 #define RT_RENDER_PERSISTENT_PATHS 32u   /* always one persistent launch */
 
