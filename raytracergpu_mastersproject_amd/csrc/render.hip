@@ -1047,7 +1047,7 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
 bool wave_schedule(uint32_t width, uint32_t height, uint32_t spp, uint32_t flags) {
     const bool layout = (flags & (RT_RENDER_SPLIT_SAMPLES | RT_RENDER_NO_SPLIT | RT_RENDER_PERSISTENT_PATHS)) != 0;
     RT_REQUIRE(!(layout && (flags & RT_RENDER_WAVEFRONT)), "RT_RENDER_WAVEFRONT excludes the persistent-launch flags");
-    return (flags & RT_RENDER_WAVEFRONT) || (!layout && (uint64_t)width * height * spp >= (1ull << 22));
+    return (flags & RT_RENDER_WAVEFRONT) || (!layout && (uint64_t)width * height * spp >= kWaveMinSamples);
 }
 
 void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,

@@ -373,8 +373,11 @@ void launch_trace_items(rt_context *ctx, const std::vector<TraceItem> &items, ui
 void launch_accum_clear(float *accum, uint32_t npix, hipStream_t stream);
 // Bounce-by-bounce rendering of one job into its image (see RT_RENDER_WAVEFRONT), with the
 // path storage from `scratch` (at least wave_scratch_bytes(width * height)) or, when it is
-// NULL, from the stream-ordered pool.
+// NULL, from the context's scratch cache.
 uint64_t wave_scratch_bytes(uint64_t paths);
+// Pixel-samples from which an image fills the device on its own and is rendered bounce by
+// bounce (by rt_render, and by the server, which then runs it alone)
+constexpr uint64_t kWaveMinSamples = 1ull << 22;
 // rt_render's choice of schedule: bounce by bounce (true) or one persistent launch
 bool wave_schedule(uint32_t width, uint32_t height, uint32_t spp, uint32_t flags);
 uint64_t per_pass_limit(uint64_t npix);  // samples of an npix-pixel image per pass

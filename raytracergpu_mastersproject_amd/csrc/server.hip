@@ -214,9 +214,10 @@ bool enqueue_step(rt_server *s) {
     // the first that does not fit gets the number of samples that does (at least one when it
     // is the step's first render).
     //
-    // A render that fills the device by itself (at least ~32 samples per resident lane) runs
-    // alone, bounce by bounce as rt_render would run it (below), so the small requests behind
-    // it are not held until the end of a launch that also carries it.  Small renders are fused
+    // A render that fills the device by itself (at least kWaveMinSamples pixel-samples left,
+    // rt_render's own threshold) runs alone, bounce by bounce as rt_render would run it
+    // (below), so the small requests behind it are not held until the end of a launch that
+    // also carries it.  Small renders are fused
     // into one persistent launch, every job in the split layout with one group size for the
     // launch, chosen so the launch has ~32 work items per resident lane: its drain (lanes
     // finishing their last items) stays short and each item traces its first segment once for
@@ -229,8 +230,7 @@ bool enqueue_step(rt_server *s) {
     std::vector<std::pair<std::shared_ptr<Request>, uint32_t>> plan;
     uint64_t total = 0;
     auto npix_of = [](const Request &r) { return (uint64_t)r.req.width * r.req.height; };
-    const uint64_t fills = 32ull * device_lanes(s);
-    auto is_big = [&](const Request &r) { return npix_of(r) * r.remaining >= fills; };
+    auto is_big = [&](const Request &r) { return npix_of(r) * r.remaining >= kWaveMinSamples; };
     bool big_step = false;
     if (!serial && !renders.empty()) {
         const bool big = is_big(*renders[0]);
