@@ -112,6 +112,59 @@ def test_edge_rays(ctx, native, O):
     ctx.release(sid)
 
 
+def _mismatch(g, o):
+    hit = o["prim"] != L.NO_HIT
+    return (g["prim"] != o["prim"]) | (hit & (g["t"].view(np.uint32) != o["t"].view(np.uint32)))
+
+
+def test_far_origins_grazing_rays_and_negative_tmin(ctx, native, O):
+    """Rays from 10 to 1e5 units away aimed into a cluster scene, half of them with a negative tmin
+    (the interval starts behind the origin), and rays in the planes of the scene box's faces.
+
+    Hits and per-ray counts equal the oracle's replay of the same traversal exactly.  Against the
+    two other oracles they agree except where fp32 itself decides differently at these distances:
+    - the reference traversal's strict slab test (AABBhitCheck, raytraceBVH.comp:184-193) can cull
+      a box the ray enters, so it misses hits brute force finds (36 of these 22,000 rays); the
+      robust slab test does not, and finds them;
+    - fp32 triangleHit can report a hit that lies outside the triangle's box (a hit point 5e4
+      units out carries ~3e-3 of rounding, against 0.005-unit triangles); brute force reports it,
+      and every traversal that tests boxes first misses it (1 ray here).
+    So every ray equals brute force or the reference traversal, and most equal both."""
+    tris, mats = small_cluster_scene(4, 2000, seed=13)
+    sid, osc = upload_both(ctx, O, tris, None, mats)
+    lo, hi = (np.asarray(x, np.float64) for x in W.scene_bounds(tris))
+    rng = W.SplitMix64(77)
+    n = 20000
+    target = lo + (hi - lo) * rng.uniform(3 * n).reshape(n, 3)
+    dirs = rng.sphere_dirs(n)
+    dist = np.exp(rng.uniform(n, np.log(10.0), np.log(1e5)))
+    rays = np.zeros(n, L.RAY)
+    rays["o"] = target - dirs * dist[:, None]
+    rays["d"] = dirs
+    rays["tmin"] = np.where(np.arange(n) % 2 == 0, 0.001, -0.5 * dist)
+    rays["tmax"] = 1e7
+    m = 2000  # origins on a face plane of the box, directions inside that plane
+    face = rng.uniform(m) < 0.5
+    pts = lo + (hi - lo) * rng.uniform(3 * m).reshape(m, 3)
+    pts[:, 1] = np.where(face, lo[1], hi[1])
+    ang = rng.uniform(m, 0.0, 2.0 * np.pi)
+    graze = np.zeros(m, L.RAY)
+    graze["o"] = pts - 3.0 * np.stack([np.cos(ang), np.zeros(m), np.sin(ang)], axis=1)
+    graze["d"] = np.stack([np.cos(ang), np.zeros(m), np.sin(ang)], axis=1)
+    graze["tmin"], graze["tmax"] = 0.001, 1e7
+    rays = np.concatenate([rays, graze])
+    g, gc = gpu_trace(ctx, sid, rays, counts=True)
+    ow, oc = osc.trace(rays, O.WIDE, counts=True)
+    assert compare_hits(g, ow) == 0
+    assert np.array_equal(gc[:, 0], oc["n_box"]) and np.array_equal(gc[:, 1], oc["n_tri"])
+    vs_brute = _mismatch(g, osc.trace(rays, O.BRUTE))
+    vs_ref = _mismatch(g, osc.trace(rays, O.REF_BVH))
+    assert not np.any(vs_brute & vs_ref)
+    assert vs_brute.sum() <= len(rays) // 1000 and vs_ref.sum() <= len(rays) // 100
+    assert np.mean(g["prim"][:n] != L.NO_HIT) > 0.01
+    ctx.release(sid)
+
+
 def test_empty_scene_misses(ctx, native):
     sid = ctx.upload(None, None, W.make_materials([1, 1, 1], L.DIFFUSE))
     rays = W.incoherent_rays(1000, [-1, -1, -1], [1, 1, 1], seed=1)
