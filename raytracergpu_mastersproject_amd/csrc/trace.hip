@@ -1,11 +1,12 @@
 // trace.hip -- ray-query kernels (closest-hit / any-hit) over the resident wide BVH.
 //
 // Two schedules:
-//   persistent: a grid sized to fill every CU once; each wave pulls rays from a
-//     global work counter.  When at least `refill` lanes of a wave are idle the
-//     wave takes that many new rays with one atomic (ballot + popcount + lane
-//     prefix), so finished lanes are refilled while long rays keep traversing.
-//   direct: one lane per ray (the plain baseline).
+//   persistent: a grid sized to fill every CU once; each wave takes chunks of rays
+//     from a global counter, one atomic per chunk (ChunkSchedule: smaller chunks for
+//     the last rays).  When at least `refill` lanes of a wave are idle they take the
+//     next rays of the wave's chunk (ballot + popcount + lane prefix), so finished
+//     lanes are refilled while long rays keep traversing.
+//   direct: one lane per ray (a launch-tuner candidate; the camera rays of renders).
 // The batched variant runs many scenes' ray batches in one persistent launch.
 #include <algorithm>
 #include <cstdlib>
