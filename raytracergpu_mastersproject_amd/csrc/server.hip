@@ -4,15 +4,16 @@
 // request's buffers fit in the first half of the server's device arena (first-fit
 // allocation in one hipMalloc made at creation, so admission never calls the allocator).
 // Admitted requests advance in scheduling steps, enqueued on the server's stream up to
-// `pipeline` steps ahead of the device.  A step is one fused render launch
-// (launch_render_jobs) plus one fused ray-query launch over slices of every admitted
-// query (launch_trace_items).  The render launch carries either the oldest render alone,
-// when it fills the device by itself, or the admitted small renders together, whole
-// where they fit the step's budget of device time (rates measured from earlier steps);
-// renders larger than the budget are cut into slices of samples.  Renders run in the
-// split layout, their per-sample storage taken from the second half of the arena for
-// the step's lifetime.  A request's last slice is followed by its resolve; when that
-// step's event completes the result is copied out and its arena range freed.
+// `pipeline` steps ahead of the device.  A step is one render launch plus one fused
+// ray-query launch over slices of every admitted query (launch_trace_items).  The render
+// launch carries either the oldest render alone, bounce by bounce as rt_render runs it,
+// when it has at least kWaveMinSamples pixel-samples left, or the admitted small renders
+// together in one persistent launch (launch_render_jobs, split layout), whole where they
+// fit the step's budget of device time (rates measured from earlier steps); renders
+// larger than the budget are cut into slices of samples.  A step's path state or
+// per-sample storage comes from the second half of the arena for the step's lifetime.
+// A request's last slice is followed by its resolve; when that step's event completes
+// the result is copied out and its arena range freed.
 //
 // A render's samples may be cut into any number of slices without changing its image
 // (each pixel's chain runs through its accumulator), and a ray's result does not
