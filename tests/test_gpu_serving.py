@@ -75,6 +75,28 @@ def test_render_batch_images_equal_single_renders(ctx, native, O, scenes):
     assert np.array_equal(alone[0].view(np.uint32), o.view(np.uint32))
 
 
+def test_render_batch_renders_a_large_image_by_default_schedule(ctx, native, scenes):
+    """An image of 2^22 pixel-samples in a batch, without flags, takes rt_render's default
+    bounce-by-bounce schedule beside a small fused job; both equal their renders alone."""
+    ids, params, _, _ = scenes
+    spec = [("ico", 512, 512, 16, 0, 21), ("soup4", 40, 30, 3, 0, 22)]
+    accs, jobs, alone = [], [], []
+    for name, w, h, spp, flags, rs in spec:
+        p = with_state(params[name], rs)
+        acc = native.DeviceArray(16 * w * h)
+        ctx.accum_clear(acc, w * h)
+        accs.append(acc)
+        jobs.append((ids[name], p, w, h, spp, acc, flags))
+        alone.append(render_alone(native, ctx, ids[name], p, w, h, spp, flags))
+    ctx.render_batch(jobs)
+    for k, ref in enumerate(alone):
+        got = accs[k].to_numpy(np.float32, ref.size).reshape(-1, 4)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), spec[k]
+    forced = render_alone(native, ctx, ids["ico"], with_state(params["ico"], 21), 512, 512, 16,
+                          native.RENDER_PERSISTENT_PATHS)
+    assert np.array_equal(alone[0].view(np.uint32), forced.view(np.uint32))
+
+
 def test_render_batch_continues_each_chain(ctx, native, scenes):
     """spp = 2 then 3 in two batches equals spp = 5 alone for every image."""
     ids, params, _, _ = scenes
