@@ -7,13 +7,15 @@ restate the reference's structs independently of the product package.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "liboracle.so"
+# RT_ORACLE_LIB: another build of rt_oracle.c (the tests' planted-mistake builds use it)
+LIB_PATH = Path(os.environ.get("RT_ORACLE_LIB", HERE / "liboracle.so"))
 
 TRIANGLE = np.dtype([("v0", "<f4", 4), ("v1", "<f4", 4), ("v2", "<f4", 4), ("materialIndex", "<u4"),
                      ("modelIndex", "<u4"), ("_pad", "<u4", 2)])
@@ -33,6 +35,7 @@ COUNTS = np.dtype([("n_box", "<u4"), ("n_tri", "<u4")])
 BRUTE, REF_BVH, WIDE = 0, 1, 2
 BUILD_REFERENCE_LEAVES, BUILD_REFERENCE_MORTON = 1, 2
 RENDER_REFERENCE_MATERIALS = 1
+TRACE_FP32 = 2
 NO_HIT = 0xFFFFFFFF
 SPHERE_BIT = 0x80000000
 
@@ -72,6 +75,9 @@ for name, res, args in [
     ("orc_triangle_hit", C.c_int, [_vp, _vp, _vp, _f, _f, _vp, _vp, _vp, _vp, _vp]),
     ("orc_sphere_hit", C.c_int, [_vp, _vp, _vp, _f, _f, _vp, _vp, _vp]),
     ("orc_aabb_hit_ref", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("orc_sphere_uv", None, [_vp, _vp, _vp]),
+    ("orc_refine_triangle", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("orc_refine_sphere", C.c_int, [_vp, _vp, _vp, _vp]),
     ("orc_model_to_world", None, [_vp, _u32, _vp, _u32, _vp]),
     ("orc_transform_mat4", None, [_vp, _vp, _vp, _vp]),
     ("orc_enclosing_aabb", None, [_vp, _u32, _vp, _u32, _vp, _vp]),
@@ -82,6 +88,7 @@ for name, res, args in [
     ("orc_render_rows", None, [C.POINTER(Scene), C.c_int, _vp, _u32, _u32, _u32, _u32, _vp, _u32, _u32,
                                C.POINTER(C.c_uint64), C.c_int]),
     ("orc_resolve", None, [_vp, _u32, _u32, _vp]),
+    ("orc_scatter", C.c_int, [_vp, _u32, _vp, _vp, C.c_int, C.POINTER(_u32), _vp, _vp]),
 ]:
     fn = getattr(lib, name)
     fn.restype = res
@@ -151,6 +158,32 @@ def sphere_hit(sph: np.ndarray, o, d, tmin, tmax):
     ok = lib.orc_sphere_hit(sph.ctypes.data, o.ctypes.data, d.ctypes.data, tmin, tmax, t.ctypes.data,
                             n.ctypes.data, bf.ctypes.data)
     return bool(ok), float(t[0]), n, int(bf[0])
+
+
+def sphere_uv(center):
+    c = _f3(center)
+    u, v = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    lib.orc_sphere_uv(c.ctypes.data, u.ctypes.data, v.ctypes.data)
+    return float(u[0]), float(v[0])
+
+
+def refine_triangle(tri, o, d, t=0.0, u=0.0, v=0.0):
+    """The refined (double-precision) t, u, v of a triangle hit; (ok, t, u, v)."""
+    tri = np.ascontiguousarray(np.asarray(tri, dtype=TRIANGLE).reshape(()))
+    tt, uu, vv = np.float32([t]), np.float32([u]), np.float32([v])
+    o, d = _f3(o), _f3(d)
+    ok = lib.orc_refine_triangle(tri.ctypes.data, o.ctypes.data, d.ctypes.data, tt.ctypes.data, uu.ctypes.data,
+                                 vv.ctypes.data)
+    return bool(ok), tt[0], uu[0], vv[0]
+
+
+def refine_sphere(sph, o, d, t):
+    """The refined t of a sphere hit: the double-precision root nearest the fp32 t; (ok, t)."""
+    sph = np.ascontiguousarray(np.asarray(sph, dtype=SPHERE).reshape(()))
+    tt = np.float32([t])
+    o, d = _f3(o), _f3(d)
+    ok = lib.orc_refine_sphere(sph.ctypes.data, o.ctypes.data, d.ctypes.data, tt.ctypes.data)
+    return bool(ok), tt[0]
 
 
 def aabb_hit_ref(o, d, bmin, bmax) -> bool:
@@ -231,12 +264,14 @@ class OracleScene:
                        len(self.mats), _p(self.nodes), len(self.nodes), _p(self.slots), len(self.slots),
                        _p(self.wide), 0 if self.wide is None else len(self.wide) // (8 * width), width, 0)
 
-    def trace(self, rays, method=BRUTE, any_hit=False, counts=False, threads=0):
+    def trace(self, rays, method=BRUTE, any_hit=False, counts=False, threads=0, fp32=False):
+        """Ray queries.  t, u, v are refined in double precision (the product's default ray-query
+        output) unless fp32 is set, which returns the reference's fp32 triangleHit values."""
         rays = np.ascontiguousarray(rays, dtype=RAY)
         hits = np.zeros(len(rays), HIT)
         cnt = np.zeros(len(rays), COUNTS) if counts else None
-        lib.orc_trace_batch(C.byref(self.c), method, _p(rays), len(rays), 1 if any_hit else 0, _p(hits), _p(cnt),
-                            threads)
+        flags = (1 if any_hit else 0) | (TRACE_FP32 if fp32 else 0)
+        lib.orc_trace_batch(C.byref(self.c), method, _p(rays), len(rays), flags, _p(hits), _p(cnt), threads)
         return (hits, cnt) if counts else hits
 
     def render(self, params, width, height, spp, accum=None, flags=0, method=BRUTE, rows=None, threads=0):
@@ -250,6 +285,17 @@ class OracleScene:
         lib.orc_render_rows(C.byref(self.c), method, params.ctypes.data, width, height, spp, flags, _p(accum), r0,
                             r1, C.byref(segs), threads)
         return accum, segs.value
+
+
+def scatter(material, d, n, backface: bool, state: int, flags: int = 0):
+    """One bounce's scatter: (continues, attenuation, out direction, state after its draws)."""
+    m = np.ascontiguousarray(np.asarray(material, dtype=MATERIAL).reshape(()))
+    d, n = _f3(d), _f3(n)
+    st = C.c_uint32(state)
+    att, out = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    ok = lib.orc_scatter(m.ctypes.data, flags, d.ctypes.data, n.ctypes.data, 1 if backface else 0, C.byref(st),
+                         att.ctypes.data, out.ctypes.data)
+    return bool(ok), att, out, st.value
 
 
 def resolve(accum, spp):

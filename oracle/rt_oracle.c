@@ -17,6 +17,11 @@
  *   orc_triangle_hit     raytraceBVH.comp:118-149    fp64 Moller-Trumbore (t,u,v 1e-5 rel.), axis-aligned
  *                                                    analytic cases, inclusive edges, parallel cutoff
  *   orc_sphere_hit       raytraceBVH.comp:152-181    analytic sphere distances
+ *   orc_sphere_uv        raytraceBVH.comp:172-173    known angles (axis centres), the v = acos(-c.y)/pi
+ *                                                    range, NaN outside |c.y| <= 1
+ *   orc_refine_*         (not in the reference: the  exact rational arithmetic (Python fractions) on
+ *                         ray queries' refined t/u/v)  random and ill-conditioned hits: within one fp32
+ *                                                    ulp of the exact value; analytic sphere roots
  *   orc_aabb_hit_ref     raytraceBVH.comp:184-193    analytic slab cases
  *   orc_build_lbvh       GetEnclosingAABB.comp:64-110, GenerateMortonCodesOfPrimitives.comp:41-94,
  *                        RadixSortSimple.comp:52-158, ConstructHLBVH.comp:58-215,
@@ -29,6 +34,13 @@
  *                                                    brute force == reference traversal == wide replay
  *   orc_render           raytraceBVH.comp:276-375    closed-form single-bounce cases (emitter seen
  *                                                    directly; miss -> 0), RNG chaining across spp
+ *   orc_scatter          raytraceBVH.comp:100-115 + the METALLIC/DIELECTRIC extension (A5):
+ *                                                    closed-form scenes decided in float64 from the
+ *                                                    pinned RNG draws (fuzz-0 mirror into an emitter,
+ *                                                    normal incidence through a slab, total internal
+ *                                                    reflection), unit-length / Snell / fuzz-cone /
+ *                                                    draw-count invariants; planted mistakes in this
+ *                                                    function fail them (test_oracle_mutants.py)
  *   orc_resolve          SingleTriangleFullScreen.frag:13-21   closed form
  *   orc_transform_mat4   TransformComponent.cpp:9-45 rotation-matrix identities (orthonormal, det=1)
  *   orc_model_to_world   ModelSpaceToWorldSpace.comp:32-48     identity / translation cases
@@ -261,6 +273,69 @@ int orc_sphere_hit(const orc_sphere *s, const float o_[3], const float d_[3], fl
     if (backface) *backface = bf;
     if (normal) st3(normal, smul((float)(1 - 2 * bf), n));
     return 1;
+}
+
+/* sphereHit's u, v (raytraceBVH.comp:172-173) depend on the centre only (A17):
+ * u = (atan(-c.z, c.x) + pi) / (2 pi), v = acos(-c.y) / pi, evaluated in double, rounded once. */
+void orc_sphere_uv(const float c[3], float *u, float *v) {
+    const double pi = 3.14159265358979323846;
+    *u = (float)((atan2(-(double)c[2], (double)c[0]) + pi) / (2.0 * pi));
+    *v = (float)(acos(-(double)c[1]) / pi);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Refined ray-query results (the product's default, not the reference's):   */
+/* t, u, v of a hit recomputed in double from the fp32 primitive and ray.      */
+/* Every double operation rounds once (built with -ffp-contract=off).          */
+/* ------------------------------------------------------------------------- */
+typedef struct { double x, y, z; } d3;
+static inline d3 dmk(const float *p) { d3 r; r.x = p[0]; r.y = p[1]; r.z = p[2]; return r; }
+static inline d3 dsub(d3 a, d3 b) { d3 r; r.x = a.x - b.x; r.y = a.y - b.y; r.z = a.z - b.z; return r; }
+static inline double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline d3 dcross(d3 a, d3 b) {
+    d3 r;
+    r.x = a.y * b.z - b.y * a.z;
+    r.y = a.z * b.x - b.z * a.x;
+    r.z = a.x * b.y - b.x * a.y;
+    return r;
+}
+
+/* Moller-Trumbore (1997) in double; u = weight of v1, v = weight of v2 (triangleHit's a, b).
+ * Returns 0 and leaves the outputs alone when the determinant is exactly zero. */
+int orc_refine_triangle(const orc_triangle *tri, const float o_[3], const float d_[3], float *t, float *u,
+                        float *v) {
+    d3 v0 = dmk(tri->v0), e1 = dsub(dmk(tri->v1), v0), e2 = dsub(dmk(tri->v2), v0);
+    d3 o = dmk(o_), d = dmk(d_);
+    d3 p = dcross(d, e2);
+    double det = ddot(e1, p);
+    if (det == 0.0) return 0;
+    d3 sv = dsub(o, v0);
+    d3 q = dcross(sv, e1);
+    *u = (float)(ddot(sv, p) / det);
+    *v = (float)(ddot(d, q) / det);
+    *t = (float)(ddot(e2, q) / det);
+    return 1;
+}
+
+/* The root of |o + t d - c|^2 = r^2 nearest the fp32 t, from the stable pair q/a, c/q
+ * (Numerical Recipes 5.6).  Returns 0 and leaves t alone for a negative discriminant. */
+int orc_refine_sphere(const orc_sphere *s, const float o_[3], const float d_[3], float *t) {
+    d3 oc = dsub(dmk(o_), dmk(s->center)), d = dmk(d_);
+    double a = ddot(d, d), hb = ddot(oc, d), r = s->radius;
+    double cc = ddot(oc, oc) - r * r;
+    double disc = hb * hb - a * cc;
+    if (!(disc >= 0.0) || !(a > 0.0)) return 0;
+    double sq = sqrt(disc);
+    double q = hb >= 0.0 ? -(hb + sq) : (sq - hb);
+    double r1 = q / a, r2 = q != 0.0 ? cc / q : r1, tf = *t;
+    *t = (float)(fabs(r1 - tf) <= fabs(r2 - tf) ? r1 : r2);
+    return 1;
+}
+
+void orc_refine_hit(const orc_scene *sc, const orc_ray *ray, orc_hit *h) {
+    if (h->prim == ORC_NO_HIT) return;
+    if (h->prim & ORC_SPHERE_BIT) orc_refine_sphere(sc->sph + (h->prim & ~ORC_SPHERE_BIT), ray->o, ray->d, &h->t);
+    else orc_refine_triangle(sc->tris + h->prim, ray->o, ray->d, &h->t, &h->u, &h->v);
 }
 
 int orc_aabb_hit_ref(const float o[3], const float d[3], const float bmin[3], const float bmax[3]) {
@@ -577,6 +652,7 @@ static int test_prim(const orc_scene *sc, uint32_t prim, v3 o, v3 d, float tmin,
     if (prim & ORC_SPHERE_BIT) {
         const orc_sphere *s = sc->sph + (prim & ~ORC_SPHERE_BIT);
         if (!orc_sphere_hit(s, oo, dd, tmin, tmax, &t, n, &bf)) return 0;
+        orc_sphere_uv(s->center, &u, &v);
         h->material = s->materialIndex;
     } else {
         const orc_triangle *tr = sc->tris + prim;
@@ -807,6 +883,7 @@ void orc_trace_batch(const orc_scene *sc, int method, const orc_ray *rays, uint3
         orc_counts c = {0, 0};
         if (flags & 1u) orc_any(sc, method, r->o, r->d, r->tmin, r->tmax, &hits[i], &c);
         else orc_closest(sc, method, r->o, r->d, r->tmin, r->tmax, &hits[i], NULL, NULL, &c);
+        if (!(flags & ORC_TRACE_FP32)) orc_refine_hit(sc, r, &hits[i]);
         if (counts) counts[i] = c;
     }
     (void)threads;
@@ -834,37 +911,52 @@ static v3 ray_color(const orc_scene *sc, int method, v3 o, v3 dir, uint32_t dept
         v3 p = add3(o, smul(h.t, d));
         v3 emitted = m->materialType == ORC_LIGHT ? alb : mk(0.0f, 0.0f, 0.0f);
         color = add3(color, mul3(emitted, atten));
-        v3 att, nd;
-        int scattered = 0;
-        if (m->materialType == ORC_DIFFUSE) {
-            att = alb;
-            v3 ruv = normalize3(rand_in_unit_sphere(st));
-            nd = normalize3(add3(h.normal, ruv));
-            scattered = 1;
-        } else if (m->materialType == ORC_METALLIC && !(flags & ORC_RENDER_REFERENCE_MATERIALS)) {
-            att = alb;
-            v3 refl = reflect3(d, h.normal);
-            v3 rs = rand_in_unit_sphere(st);
-            nd = normalize3(add3(refl, smul(m->albedo[3], rs)));
-            scattered = dot3(nd, h.normal) > 0.0f;
-        } else if (m->materialType == ORC_DIELECTRIC && !(flags & ORC_RENDER_REFERENCE_MATERIALS)) {
-            att = mk(1.0f, 1.0f, 1.0f);
-            float ior = m->albedo[3];
-            float ri = h.backface ? ior : (1.0f / ior); /* front face: entering the medium */
-            float ct = gmin(dot3(neg3(d), h.normal), 1.0f);
-            float stt = sqrtf(1.0f - ct * ct);
-            int cannot = ri * stt > 1.0f;
-            if (cannot || schlick(ct, ri) > orc_rng_next(st)) nd = reflect3(d, h.normal);
-            else nd = refract3(d, h.normal, ri);
-            nd = normalize3(nd);
-            scattered = 1;
-        }
-        if (!scattered) break;
-        atten = mul3(atten, att);
+        float dd[3], nn[3], att[3], nd[3];
+        st3(dd, d);
+        st3(nn, h.normal);
+        if (!orc_scatter(m, flags, dd, nn, h.backface, st, att, nd)) break;
+        atten = mul3(atten, ld3(att));
         o = p;
-        d = nd;
+        d = ld3(nd);
     }
     return color;
+}
+
+/* scatter() of raytraceBVH.comp:100-115 (DIFFUSE; LIGHT and, with ORC_RENDER_REFERENCE_MATERIALS,
+ * METALLIC/DIELECTRIC absorb) and the textbook METALLIC/DIELECTRIC of Ray Tracing in One Weekend
+ * (secs. 10-11), the extension A5: fuzzed mirror reflection, absorbed when it points into the surface;
+ * Snell refraction with Schlick's reflectance, total internal reflection drawing no random number. */
+int orc_scatter(const orc_material *m, uint32_t flags, const float d_[3], const float n_[3], int backface,
+                uint32_t *st, float att_[3], float out[3]) {
+    v3 d = ld3(d_), n = ld3(n_), alb = ld3(m->albedo);
+    v3 att = mk(0.0f, 0.0f, 0.0f), nd = mk(0.0f, 0.0f, 0.0f);
+    int scattered = 0;
+    if (m->materialType == ORC_DIFFUSE) {
+        att = alb;
+        v3 ruv = normalize3(rand_in_unit_sphere(st));
+        nd = normalize3(add3(n, ruv));
+        scattered = 1;
+    } else if (m->materialType == ORC_METALLIC && !(flags & ORC_RENDER_REFERENCE_MATERIALS)) {
+        att = alb;
+        v3 refl = reflect3(d, n);
+        v3 rs = rand_in_unit_sphere(st);
+        nd = normalize3(add3(refl, smul(m->albedo[3], rs)));
+        scattered = dot3(nd, n) > 0.0f;
+    } else if (m->materialType == ORC_DIELECTRIC && !(flags & ORC_RENDER_REFERENCE_MATERIALS)) {
+        att = mk(1.0f, 1.0f, 1.0f);
+        float ior = m->albedo[3];
+        float ri = backface ? ior : (1.0f / ior); /* front face: entering the medium */
+        float ct = gmin(dot3(neg3(d), n), 1.0f);
+        float stt = sqrtf(1.0f - ct * ct);
+        int cannot = ri * stt > 1.0f;
+        if (cannot || schlick(ct, ri) > orc_rng_next(st)) nd = reflect3(d, n);
+        else nd = refract3(d, n, ri);
+        nd = normalize3(nd);
+        scattered = 1;
+    }
+    st3(att_, att);
+    st3(out, nd);
+    return scattered;
 }
 
 void orc_render_rows(const orc_scene *sc, int method, const orc_params *p, uint32_t width, uint32_t height,

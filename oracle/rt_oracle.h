@@ -105,6 +105,13 @@ int orc_triangle_hit(const orc_triangle *tri, const float o[3], const float d[3]
 int orc_sphere_hit(const orc_sphere *s, const float o[3], const float d[3], float tmin, float tmax,
                    float *t, float normal[3], int *backface);
 int orc_aabb_hit_ref(const float o[3], const float d[3], const float bmin[3], const float bmax[3]);
+/* raytraceBVH.comp:172-173: a sphere hit's u, v from its centre */
+void orc_sphere_uv(const float center[3], float *u, float *v);
+/* refined results (the product's default ray-query output; no reference counterpart):
+ * t/u/v recomputed in double and rounded once; 0 = no solution in double, outputs untouched */
+int orc_refine_triangle(const orc_triangle *tri, const float o[3], const float d[3], float *t, float *u,
+                        float *v);
+int orc_refine_sphere(const orc_sphere *s, const float o[3], const float d[3], float *t);
 
 /* ---- LBVH builder (GetEnclosingAABB, GenerateMortonCodesOfPrimitives, RadixSortSimple,
  *      ConstructHLBVH, ConstructAABBsOfInternalNodes) ---- */
@@ -149,7 +156,9 @@ uint32_t orc_closest(const orc_scene *sc, int method, const float o[3], const fl
 int orc_any(const orc_scene *sc, int method, const float o[3], const float d[3], float tmin,
             float tmax, orc_hit *hit, orc_counts *counts);
 
-/* batch ray queries; flags bit0 = any-hit. counts may be NULL. threads <= 0 -> all cores */
+/* batch ray queries; flags: bit0 = any-hit, ORC_TRACE_FP32 = the fp32 triangleHit t/u/v (default:
+ * refined as orc_refine_*). counts may be NULL. threads <= 0 -> all cores */
+#define ORC_TRACE_FP32 2u
 void orc_trace_batch(const orc_scene *sc, int method, const orc_ray *rays, uint32_t n, uint32_t flags,
                      orc_hit *hits, orc_counts *counts, int threads);
 
@@ -165,6 +174,11 @@ void orc_render_rows(const orc_scene *sc, int method, const orc_params *p, uint3
                      uint32_t row1, uint64_t *segments, int threads);
 /* SingleTriangleFullScreen.frag:13-21 */
 void orc_resolve(const float *accum, uint32_t npix, uint32_t spp, float *rgba_out);
+/* one bounce's scatter (raytraceBVH.comp:100-115 + the METALLIC/DIELECTRIC extension): d the unit
+ * incoming direction, n the face-forwarded unit normal; returns 1 when the path continues along out
+ * with attenuation att.  Draws from *state as the path tracer does. */
+int orc_scatter(const orc_material *m, uint32_t flags, const float d[3], const float n[3], int backface,
+                uint32_t *state, float att[3], float out[3]);
 
 #ifdef __cplusplus
 }
