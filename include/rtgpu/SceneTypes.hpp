@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "../rtgpu.h"
 
@@ -16,6 +17,11 @@ struct vec3 {
     f32 x = 0.0f, y = 0.0f, z = 0.0f;
     constexpr vec3() = default;
     constexpr vec3(f32 a, f32 b, f32 c) : x(a), y(b), z(c) {}
+    // any three arithmetic values, each converted to f32 (glm's vec3(X, Y, Z) constructor)
+    template <typename A, typename B, typename C,
+              typename = std::enable_if_t<std::is_arithmetic_v<A> && std::is_arithmetic_v<B> &&
+                                          std::is_arithmetic_v<C>>>
+    constexpr vec3(A a, B b, C c) : x(static_cast<f32>(a)), y(static_cast<f32>(b)), z(static_cast<f32>(c)) {}
     constexpr explicit vec3(f32 s) : x(s), y(s), z(s) {}
     constexpr f32 operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
     bool operator==(const vec3 &o) const { return x == o.x && y == o.y && z == o.z; }

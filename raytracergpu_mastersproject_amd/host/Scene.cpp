@@ -105,7 +105,11 @@ std::string modelPath(const std::string &filepath) {
 }
 
 // OBJ parsing: `v x y z [w]` and `f` records with v, v/vt, v/vt/vn or v//vn references (1-based,
-// negative = relative to the end).  Polygons are split into fans (v0, vi, vi+1).
+// negative = relative to the end).  The reference loads meshes with tinyobjloader's LoadObj, which
+// triangulates by default (RTModel.cpp:53); its triangulation of a quad (exportGroupsToShape in the
+// library's 2.x releases) splits along the shorter diagonal, with the squared lengths summed in
+// float: (0,1,2),(0,2,3) when |v2-v0|^2 < |v3-v1|^2, else (0,1,3),(1,2,3).  Larger polygons, which
+// that library ear-clips in a projection plane, are split into fans (v0, vi, vi+1) here.
 std::vector<SceneTypes::CPU::Triangle> loadObjTriangles(const std::string &filepath) {
     const std::string path = modelPath(filepath);
     std::ifstream in(path);
@@ -132,6 +136,20 @@ std::vector<SceneTypes::CPU::Triangle> loadObjTriangles(const std::string &filep
                 if (v < 1 || v > (long)verts.size())
                     throw std::runtime_error(path + ":" + std::to_string(lineno) + ": vertex index out of range");
                 idx.push_back(v - 1);
+            }
+            if (idx.size() == 4) {
+                const vec3 &a = verts[idx[0]], &b = verts[idx[1]], &c = verts[idx[2]], &d = verts[idx[3]];
+                const f32 ex = c.x - a.x, ey = c.y - a.y, ez = c.z - a.z;
+                const f32 fx = d.x - b.x, fy = d.y - b.y, fz = d.z - b.z;
+                const f32 sqr02 = ex * ex + ey * ey + ez * ez, sqr13 = fx * fx + fy * fy + fz * fz;
+                if (sqr02 < sqr13) {
+                    tris.push_back(SceneTypes::CPU::Triangle{a, b, c});
+                    tris.push_back(SceneTypes::CPU::Triangle{a, c, d});
+                } else {
+                    tris.push_back(SceneTypes::CPU::Triangle{a, b, d});
+                    tris.push_back(SceneTypes::CPU::Triangle{b, c, d});
+                }
+                continue;
             }
             for (size_t k = 1; k + 1 < idx.size(); ++k)
                 tris.push_back(SceneTypes::CPU::Triangle{verts[idx[0]], verts[idx[k]], verts[idx[k + 1]]});
