@@ -84,7 +84,7 @@ typedef struct rt_params {
     uint32_t _pad;
 } rt_params;
 
-/* ray query records: 32 B ray, 16 B hit */
+/* ray query records: 32 B ray, 16 B hit (prim = RT_NO_HIT when nothing is hit; t, u, v: see RT_TRACE_FP32) */
 typedef struct rt_ray { float o[3]; float tmin; float d[3]; float tmax; } rt_ray;
 typedef struct rt_hit { uint32_t prim; float t; float u; float v; } rt_hit;
 
@@ -147,6 +147,14 @@ int rt_scene_get_wide(rt_context *ctx, uint32_t scene_id, void *host_out, size_t
 
 #define RT_TRACE_CLOSEST 0u
 #define RT_TRACE_ANY 1u        /* any-hit: terminate at the first hit found in [tmin, tmax] */
+/* Which primitive a ray hits is decided by the reference's fp32 triangleHit / sphereHit
+ * (raytraceBVH.comp:118-181).  By default a hit's t, u, v are then recomputed once in double
+ * precision from the primitive's stored fp32 geometry and the fp32 ray (Moller-Trumbore for
+ * triangles, the nearest root of the quadratic for spheres) and rounded to fp32, so they lie
+ * within an fp32 rounding of the exact intersection.  RT_TRACE_FP32 returns the fp32 values
+ * of triangleHit itself instead (bit for bit the reference's arithmetic).  A sphere hit's u, v
+ * are the reference's centre-based coordinates (raytraceBVH.comp:172-173) either way. */
+#define RT_TRACE_FP32 2u
 
 typedef struct rt_launch_config {
     uint32_t block;        /* threads per block: 64, 128, 256 (0 = tuned default)       */

@@ -43,6 +43,17 @@ public:
     void mainLoop(uint32_t frames = 1);
     void doIteration(float frameTime);
 
+    // Config::RunRayPerPixelIncreasingDemo (Config.hpp:17-23, RaytracerBVH.hpp:522-570): frames from
+    // `startRaysPerPixel`, raising the rays per pixel by `increaseAmount` every `runsBeforeIncrease` frames
+    // until they exceed `maxRaysPerPixel`, then one line per level "index, mean frame time,\n" written to
+    // `csvPath` (the reference's runtimes.csv; times in whole microseconds with the suffix "us", as a
+    // std::chrono::microseconds prints).  Frame times are measured between successive loop starts, as
+    // the reference's are.  Returns the number of levels written.
+    struct SppSweep {
+        uint32_t runsBeforeIncrease = 4, startRaysPerPixel = 100, maxRaysPerPixel = 200, increaseAmount = 5;
+    };
+    size_t sppSweep(const SppSweep &cfg, const std::string &csvPath = "runtimes.csv");
+
     void setRandomState(uint32_t s) { randomState = s; }
     void setPrintTimings(bool on) { printTimings = on; }
     RaytraceScene &getScene() { return *scene; }

@@ -703,26 +703,50 @@ static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float 
     wentry stack[1024];
     int sp = 0;
     if (sc->nwide == 0) return ORC_NO_HIT;
-    stack[sp].e = sc->wide_root;
-    stack[sp].tn = tmin;
-    sp++;
-    while (sp > 0) {
-        wentry cur = stack[--sp];
-        if (cur.tn > bt * kRobust) continue;
-        if (cur.e & WIDE_LEAF) {
-            uint32_t first = cur.e & 0x07FFFFFFu, count = ((cur.e >> 27) & 15u) + 1u;
-            for (uint32_t s = first; s < first + count; ++s) {
-                hitrec h;
-                if (cnt) cnt->n_tri++;
-                if (test_prim(sc, sc->slot_prim[s], o, d, tmin, bt, &h)) {
-                    if (h.t < bt || (h.t == bt && s < bslot)) {
-                        bt = h.t;
-                        bslot = s;
-                        *best = h;
-                        if (any) return bslot;
+    /* The kernel's per-lane state machine (traverse.hpp): a current entry, for closest hit one leaf
+     * set aside (`pend`) while node work goes on, and the stack.  A node step or a leaf step ends by
+     * popping when nothing is current, then setting aside a current leaf when no leaf is set aside
+     * yet (and popping again).  A leaf step tests the set-aside leaf, then a current leaf. */
+    wentry cur = {sc->wide_root, tmin}, pend = {0u, 0.0f};
+    int have = 1, has_pend = 0;
+#define ORC_POP_AND_STASH()                                                       \
+    do {                                                                          \
+        if (!have && sp > 0) { cur = stack[--sp]; have = 1; }                     \
+        if (!any && have && (cur.e & WIDE_LEAF) && !has_pend) {                   \
+            pend = cur;                                                           \
+            has_pend = 1;                                                         \
+            have = 0;                                                             \
+            if (sp > 0) { cur = stack[--sp]; have = 1; }                          \
+        }                                                                         \
+    } while (0)
+    while (have || has_pend) {
+        if (!have || (cur.e & WIDE_LEAF)) { /* leaf step */
+            wentry lv[2];
+            int nl = 0;
+            if (has_pend) { lv[nl++] = pend; has_pend = 0; }
+            if (have) { lv[nl++] = cur; have = 0; }
+            for (int k = 0; k < nl; ++k) {
+                if (lv[k].tn > bt * kRobust) continue;
+                uint32_t first = lv[k].e & 0x07FFFFFFu, count = ((lv[k].e >> 27) & 15u) + 1u;
+                for (uint32_t s = first; s < first + count; ++s) {
+                    hitrec h;
+                    if (cnt) cnt->n_tri++;
+                    if (test_prim(sc, sc->slot_prim[s], o, d, tmin, bt, &h)) {
+                        if (h.t < bt || (h.t == bt && s < bslot)) {
+                            bt = h.t;
+                            bslot = s;
+                            *best = h;
+                            if (any) return bslot;
+                        }
                     }
                 }
             }
+            ORC_POP_AND_STASH();
+            continue;
+        }
+        if (cur.tn > bt * kRobust) { /* node culled by a closer hit found since it was pushed */
+            have = 0;
+            ORC_POP_AND_STASH();
             continue;
         }
         const uint32_t *nd = nodes + (size_t)stride * cur.e;
@@ -759,14 +783,18 @@ static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float 
             for (int i = 0; i < nh; ++i) cand[i].tn = -INFINITY; /* any-hit entries are never culled */
         }
         /* the nearest becomes the current entry at once, with its full distance; the others go
-         * through the kernel's stack */
-        for (int i = nh - 1; i >= 0; --i) {
+         * through the kernel's stack, farthest first */
+        for (int i = nh - 1; i >= 1; --i) {
             wentry e = cand[i];
-            if (i > 0 && !any) e.tn = stacked_tn(e.tn);
+            if (!any) e.tn = stacked_tn(e.tn);
             stack[sp++] = e;
         }
+        if (nh > 0) cur = cand[0];
+        else have = 0;
         if (sp > 1000) abort();
+        ORC_POP_AND_STASH();
     }
+#undef ORC_POP_AND_STASH
     return bslot;
 }
 

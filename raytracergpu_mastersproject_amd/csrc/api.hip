@@ -97,7 +97,7 @@ int rt_scene_release(rt_context *ctx, uint32_t scene_id) {
         if (it->second->users)
             throw Error(RT_ERR_BUSY, "scene " + std::to_string(scene_id) + " is used by " +
                                          std::to_string(it->second->users) + " server request(s)");
-        RT_HIP(hipDeviceSynchronize());  // in-flight work may still read the scene
+        it->second->wait_unused();  // work enqueued on any stream may still read the scene
         ctx->scenes.erase(it);
     });
 }
@@ -148,10 +148,11 @@ int rt_trace(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n,
              uint32_t flags, const rt_launch_config *cfg, void *stream) {
     if (!ctx) return RT_ERR_INVALID;
     return guarded(ctx, [&] {
-        RT_REQUIRE((flags & ~RT_TRACE_ANY) == 0, "unknown trace flags");
-        const Scene &s = ctx->scene(scene_id);
+        RT_REQUIRE((flags & ~kTraceFlags) == 0, "unknown trace flags");
+        Scene &s = ctx->scene(scene_id);
         LaunchCfg c = resolve_cfg(ctx, s, cfg, (flags & RT_TRACE_ANY) != 0);
         launch_trace(ctx, s, rays, n, hits, counts, flags, c, static_cast<hipStream_t>(stream));
+        s.mark_use(static_cast<hipStream_t>(stream));
     });
 }
 
@@ -166,10 +167,11 @@ int rt_render(rt_context *ctx, uint32_t scene_id, const rt_params *params, uint3
     if (!ctx || !params) return RT_ERR_INVALID;
     return guarded(ctx, [&] {
         RT_REQUIRE((flags & ~kRenderFlags) == 0, "unknown render flags");
-        const Scene &s = ctx->scene(scene_id);
+        Scene &s = ctx->scene(scene_id);
         LaunchCfg c = resolve_cfg(ctx, s, cfg, false);
         launch_render(ctx, s, *params, width, height, spp, flags, accum, segments, c,
                       static_cast<hipStream_t>(stream));
+        s.mark_use(static_cast<hipStream_t>(stream));
     });
 }
 
@@ -220,6 +222,7 @@ int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, 
         }
         for (auto &kv : by_width) launch_render_jobs(ctx, kv.second, kv.first, segments, c, st);
         for (auto &[s, rj] : waves) launch_render_wave(ctx, *s, rj, segments, c, st, nullptr, 0);
+        for (uint32_t i = 0; i < njobs; ++i) ctx->scene(jobs[i].scene_id).mark_use(st);
     });
 }
 
@@ -227,12 +230,13 @@ int rt_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems,
                    const rt_launch_config *cfg, void *stream) {
     if (!ctx) return RT_ERR_INVALID;
     return guarded(ctx, [&] {
-        RT_REQUIRE((flags & ~RT_TRACE_ANY) == 0, "unknown trace flags");
+        RT_REQUIRE((flags & ~kTraceFlags) == 0, "unknown trace flags");
         LaunchCfg c;
         const bool any = (flags & RT_TRACE_ANY) != 0;
         if (nitems) c = resolve_cfg(ctx, ctx->scene(items[0].scene_id), cfg, any);
         if (!cfg && ctx->has_batch_tuned) c = any ? ctx->batch_any : ctx->batch_closest;
         launch_trace_batch(ctx, items, nitems, flags, c, static_cast<hipStream_t>(stream));
+        for (uint32_t i = 0; i < nitems; ++i) ctx->scene(items[i].scene_id).mark_use(static_cast<hipStream_t>(stream));
     });
 }
 

@@ -387,8 +387,9 @@ __global__ void k_slots(const uint32_t *sorted_vals, uint32_t n, uint32_t ntri, 
 }
 
 // leaf boxes (ConstructHLBVH.comp:33-56, 134-169), leaf records and material table
-__device__ inline void leaf_body(const rt_triangle *tris, const rt_sphere *sph, const uint32_t *slot_prim, uint32_t i,
-                                 rt_bvh_node *leaf_nodes, LeafRecord *records, uint32_t *slot_mat) {
+__device__ inline void leaf_body(const rt_triangle *tris, const rt_sphere *sph, const float2 *sph_uv,
+                                 const uint32_t *slot_prim, uint32_t i, rt_bvh_node *leaf_nodes, LeafRecord *records,
+                                 uint32_t *slot_mat) {
     uint32_t p = slot_prim[i];
     rt_bvh_node nd;
     LeafRecord rec;
@@ -402,7 +403,8 @@ __device__ inline void leaf_body(const rt_triangle *tris, const rt_sphere *sph, 
         nd.primitiveIndex = p & ~RT_SPHERE_BIT;
         nd.primitiveType = RT_SPHERE_PRIMITIVE;
         rec.r0 = make_float4(s.center[0], s.center[1], s.center[2], s.radius);
-        rec.r1 = make_float4(__uint_as_float(kSphereTag), 0.0f, 0.0f, 0.0f);
+        const float2 uv = sph_uv[p & ~RT_SPHERE_BIT];
+        rec.r1 = make_float4(__uint_as_float(kSphereTag), uv.x, uv.y, 0.0f);
         rec.r2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         rec.r3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         slot_mat[i] = s.materialIndex;
@@ -429,16 +431,16 @@ __device__ inline void leaf_body(const rt_triangle *tris, const rt_sphere *sph, 
     if (leaf_nodes) leaf_nodes[i] = nd;
     records[i] = rec;
 }
-__global__ void k_leaves(const rt_triangle *tris, const rt_sphere *sph, const uint32_t *slot_prim, uint32_t n,
-                         rt_bvh_node *leaf_nodes, LeafRecord *records, uint32_t *slot_mat) {
+__global__ void k_leaves(const rt_triangle *tris, const rt_sphere *sph, const float2 *sph_uv, const uint32_t *slot_prim,
+                         uint32_t n, rt_bvh_node *leaf_nodes, LeafRecord *records, uint32_t *slot_mat) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) leaf_body(tris, sph, slot_prim, i, leaf_nodes, records, slot_mat);
+    if (i < n) leaf_body(tris, sph, sph_uv, slot_prim, i, leaf_nodes, records, slot_mat);
 }
 // leaf records only (caller-supplied node buffer: boxes come from that buffer)
-__global__ void k_records(const rt_triangle *tris, const rt_sphere *sph, const uint32_t *slot_prim, uint32_t n,
-                          LeafRecord *records, uint32_t *slot_mat) {
+__global__ void k_records(const rt_triangle *tris, const rt_sphere *sph, const float2 *sph_uv, const uint32_t *slot_prim,
+                          uint32_t n, LeafRecord *records, uint32_t *slot_mat) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) leaf_body(tris, sph, slot_prim, i, nullptr, records, slot_mat);
+    if (i < n) leaf_body(tris, sph, sph_uv, slot_prim, i, nullptr, records, slot_mat);
 }
 
 // ConstructAABBsOfInternalNodes.comp:27-65 computes each internal box as the union of
@@ -742,6 +744,15 @@ void build_scene(rt_context *ctx, Scene &sc, const rt_triangle *tris, uint32_t n
         k_model_to_world<<<grid_for(n), kB, 0, st>>>(dtris.as<rt_triangle>(), ntri, dsph.as<rt_sphere>(), nsph,
                                                      dmodels.as<float>());
     }
+    // centre-based u, v of every sphere (raytraceBVH.comp:172-173), from its world-space centre
+    DevBuf duv;
+    if (nsph) {
+        std::vector<rt_sphere> world(sph, sph + nsph);
+        if (models) RT_HIP(hipMemcpy(world.data(), dsph.p, sizeof(rt_sphere) * nsph, hipMemcpyDeviceToHost));
+        std::vector<float2> uv(nsph);
+        for (uint32_t i = 0; i < nsph; ++i) sphere_uv(world[i].center, uv[i].x, uv[i].y);
+        upload(duv, uv.data(), nsph);
+    }
     sc.slot_prim.alloc(sizeof(uint32_t) * n);
     sc.slot_mat.alloc(sizeof(uint32_t) * n);
     sc.records.alloc(sizeof(LeafRecord) * n);
@@ -770,8 +781,9 @@ void build_scene(rt_context *ctx, Scene &sc, const rt_triangle *tris, uint32_t n
         RT_HIP(hipMemcpy(binary, nodes, sizeof(rt_bvh_node) * (2ull * n - 1), hipMemcpyHostToDevice));
         RT_HIP(hipMemcpy(sc.slot_prim.p, hslots.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
         RT_HIP(hipMemcpy(range, hrange.data(), sizeof(uint2) * hrange.size(), hipMemcpyHostToDevice));
-        k_records<<<grid_for(n), kB, 0, st>>>(dtris.as<rt_triangle>(), dsph.as<rt_sphere>(), sc.slot_prim.as<uint32_t>(),
-                                              n, sc.records.as<LeafRecord>(), sc.slot_mat.as<uint32_t>());
+        k_records<<<grid_for(n), kB, 0, st>>>(dtris.as<rt_triangle>(), dsph.as<rt_sphere>(), duv.as<float2>(),
+                                              sc.slot_prim.as<uint32_t>(), n, sc.records.as<LeafRecord>(),
+                                              sc.slot_mat.as<uint32_t>());
     } else {
         const uint64_t mk = tmp.mark();
         uint32_t *bkeys = tmp.take<uint32_t>(6);
@@ -787,8 +799,9 @@ void build_scene(rt_context *ctx, Scene &sc, const rt_triangle *tris, uint32_t n
         if (n > 1) k_karras<<<grid_for(n - 1), kB, 0, st>>>(keys, (int)n, binary, parent, range);
         k_slots<<<grid_for(n), kB, 0, st>>>(vals, n, ntri, (opts.flags & RT_BUILD_REFERENCE_LEAVES) ? 1u : 0u,
                                             sc.slot_prim.as<uint32_t>());
-        k_leaves<<<grid_for(n), kB, 0, st>>>(dtris.as<rt_triangle>(), dsph.as<rt_sphere>(), sc.slot_prim.as<uint32_t>(), n,
-                                             binary + (n - 1), sc.records.as<LeafRecord>(), sc.slot_mat.as<uint32_t>());
+        k_leaves<<<grid_for(n), kB, 0, st>>>(dtris.as<rt_triangle>(), dsph.as<rt_sphere>(), duv.as<float2>(),
+                                             sc.slot_prim.as<uint32_t>(), n, binary + (n - 1),
+                                             sc.records.as<LeafRecord>(), sc.slot_mat.as<uint32_t>());
         if (n > 1) refit_levels(binary, parent, n, st, tmp);
         tmp.release(mk);
     }
@@ -832,6 +845,8 @@ void build_scene(rt_context *ctx, Scene &sc, const rt_triangle *tris, uint32_t n
     sc.stack_bound = (W - 1u) * depth + 1u;
     sc.build_ms = ms;
     if (keep_binary) sc.binary = std::move(kept);
+    sc.tris = std::move(dtris);  // world space: the ray queries' refinement reads the vertices
+    sc.spheres = std::move(dsph);
 }
 
 }  // namespace rtgpu

@@ -348,6 +348,7 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
     ps.job = 0;
     Trav t;
     t.have = false;
+    t.has_pend = false;
     t.top = 0;
     uint32_t segs = 0;
     bool exhausted = false;
@@ -679,6 +680,7 @@ template <int W, int S> __global__ __launch_bounds__(256) void k_wave_finish(Fin
     v3 color{0.0f, 0.0f, 0.0f}, atten{1.0f, 1.0f, 1.0f};
     Trav t;
     t.have = false;
+    t.has_pend = false;
     t.top = 0;
     uint32_t segs = 0;  // continuation segments (the queue's own were counted when queued)
     bool exhausted = false;
@@ -1052,21 +1054,33 @@ bool wave_schedule(uint32_t width, uint32_t height, uint32_t spp, uint32_t flags
 
 void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,
                    uint32_t flags, float *accum, unsigned long long *segments, const LaunchCfg &c,
-                   hipStream_t stream) {
+                   hipStream_t stream, char *scratch, uint64_t scratch_bytes) {
     RenderJob j = make_render_job(sc, p, width, height, spp, flags, accum);
     if (spp == 0) return;
     check_stack(sc, c);
     if (wave_schedule(width, height, spp, flags)) {
-        launch_render_wave(ctx, sc, j, segments, c, stream, nullptr, 0);
+        if (scratch && scratch_bytes < wave_scratch_bytes((uint64_t)width * height))
+            throw Error(RT_ERR_BUDGET, "one sample of a " + std::to_string(width) + "x" + std::to_string(height) +
+                                           " image needs " + std::to_string(wave_scratch_bytes((uint64_t)width * height)) +
+                                           " bytes of path storage; the scratch holds " + std::to_string(scratch_bytes));
+        launch_render_wave(ctx, sc, j, segments, c, stream, scratch, scratch_bytes);
         return;
     }
     float4 *samples = nullptr;
-    const uint32_t group = split_group(ctx, width, height, spp, flags);
+    uint32_t group = split_group(ctx, width, height, spp, flags);
+    if (scratch && group && 16ull * width * height * spp > scratch_bytes) {
+        // the per-sample storage does not fit: whole pixels per lane give the same image
+        if (flags & RT_RENDER_SPLIT_SAMPLES)
+            throw Error(RT_ERR_BUDGET, "RT_RENDER_SPLIT_SAMPLES needs " + std::to_string(16ull * width * height * spp) +
+                                           " bytes of sample storage; the scratch holds " +
+                                           std::to_string(scratch_bytes));
+        group = 0;
+    }
     const bool split = group != 0;
     std::unique_ptr<ScratchLease> lease;  // the samples' storage, back in the cache after the sum
     if (split) {
-        lease = std::make_unique<ScratchLease>(ctx->scratch, 16ull * width * height * spp, stream);
-        samples = reinterpret_cast<float4 *>(lease->ptr());
+        if (!scratch) lease = std::make_unique<ScratchLease>(ctx->scratch, 16ull * width * height * spp, stream);
+        samples = reinterpret_cast<float4 *>(scratch ? scratch : lease->ptr());
         make_split(j, samples, group);
     }
     RenderArgs a{};

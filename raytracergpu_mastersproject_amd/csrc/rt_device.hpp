@@ -9,6 +9,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 namespace rtgpu {
@@ -139,7 +140,8 @@ __host__ __device__ inline float schlick(float cosine, float ri) {
 //     with e1 = v1-v0, e2 = v2-v0, n = normalize(cross(e1,e2)), D = dot(n,v0),
 //     w = cross(e1,e2)/dot(cross,cross): the ray-independent part of
 //     triangleHit (raytraceBVH.comp:119-125), precomputed once.
-//   sphere:   r0 = {center, radius}, r1.x = kSphereTag
+//   sphere:   r0 = {center, radius}, r1 = {kSphereTag, u, v, 0} with u, v the
+//     centre-based coordinates of sphereHit (raytraceBVH.comp:172-173), one pair per sphere
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSphereTag = 0xFFFFFFFFu;
 
@@ -214,6 +216,71 @@ __device__ inline bool sphere_test(const LeafRecord &r, v3 o, v3 d, float tmin, 
 }
 
 __device__ inline bool is_sphere_record(const LeafRecord &r) { return __float_as_uint(r.r1.x) == kSphereTag; }
+
+// sphereHit's u, v (raytraceBVH.comp:172-173) are computed from the sphere's centre, not the
+// hit point, so they are one pair per sphere: u = (atan(-c.z, c.x) + pi) / (2 pi),
+// v = acos(-c.y) / pi, here evaluated in double and rounded once (NaN when |c.y| > 1, as acos).
+inline void sphere_uv(const float c[3], float &u, float &v) {
+    const double pi = 3.14159265358979323846;
+    u = (float)((atan2(-(double)c[2], (double)c[0]) + pi) / (2.0 * pi));
+    v = (float)(acos(-(double)c[1]) / pi);
+}
+
+// ---------------------------------------------------------------------------
+// Refined ray-query results.  triangleHit decides which primitive a ray hits, in fp32, as
+// the reference does; its t, u, v carry the rounding of that plane-based form (on
+// 0.005-unit triangles seen from a unit away, ~1e-5 relative).  A ray query then
+// recomputes them once per hit ray in double precision from the stored fp32 vertices and
+// the fp32 ray, and rounds once: within half an fp32 ulp of the exact intersection
+// except where double rounding or the conditioning of the triangle intervene.
+//   triangle: Moller-Trumbore (Moller & Trumbore 1997), u = weight of v1, v = weight of v2,
+//             the convention of triangleHit's a and b;
+//   sphere:   the root of the half-b quadratic nearest the fp32 t, with the numerically
+//             stable pair q/a, c/q (Press et al., Numerical Recipes 5.6).
+// Every double operation rounds once (no contraction), so the CPU restatement reproduces
+// the results bit for bit.
+// ---------------------------------------------------------------------------
+struct d3 {
+    double x, y, z;
+};
+__host__ __device__ inline d3 dsub(d3 a, d3 b) { return d3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline d3 dcross(d3 a, d3 b) {
+    return d3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+__host__ __device__ inline d3 dvec(float x, float y, float z) { return d3{(double)x, (double)y, (double)z}; }
+
+// false (outputs untouched) when the determinant is exactly zero
+__host__ __device__ inline bool refine_triangle(d3 v0, d3 v1, d3 v2, d3 o, d3 d, float &t, float &u, float &v) {
+    const d3 e1 = dsub(v1, v0), e2 = dsub(v2, v0);
+    const d3 p = dcross(d, e2);
+    const double det = ddot(e1, p);
+    if (det == 0.0) return false;
+    const d3 s = dsub(o, v0);
+    const d3 q = dcross(s, e1);
+    u = (float)(ddot(s, p) / det);
+    v = (float)(ddot(d, q) / det);
+    t = (float)(ddot(e2, q) / det);
+    return true;
+}
+
+// false (t untouched) when the discriminant is negative in double precision
+__host__ __device__ inline bool refine_sphere(d3 c, double radius, d3 o, d3 d, float &t) {
+    const d3 oc = dsub(o, c);
+    const double a = ddot(d, d);
+    const double hb = ddot(oc, d);
+    const double cc = ddot(oc, oc) - radius * radius;
+    const double disc = hb * hb - a * cc;
+    if (!(disc >= 0.0) || !(a > 0.0)) return false;
+    const double sq = sqrt(disc);
+    const double q = hb >= 0.0 ? -(hb + sq) : (sq - hb);
+    const double r1 = q / a;
+    const double r2 = q != 0.0 ? cc / q : r1;
+    const double tf = (double)t;
+    const double pick = fabs(r1 - tf) <= fabs(r2 - tf) ? r1 : r2;
+    t = (float)pick;
+    return true;
+}
 
 // ---------------------------------------------------------------------------
 // Wide BVH node: W children, SoA inside the node so a lane fetches its node with

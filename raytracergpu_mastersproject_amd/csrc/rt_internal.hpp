@@ -86,11 +86,51 @@ struct Scene {
     DevBuf slot_mat;   // nprim * u32: leaf slot -> material index
     DevBuf materials;  // nmat * rt_material
     DevBuf binary;     // (2N-1) * rt_bvh_node when RT_BUILD_KEEP_BINARY
+    DevBuf tris;       // ntri * rt_triangle, world space (the refined ray-query results read them)
+    DevBuf spheres;    // nsph * rt_sphere, world space
     bool has_tuned = false;
     LaunchCfg tuned_closest, tuned_any;
     uint32_t users = 0;  // server requests holding this scene
+    // The last work enqueued on each stream that read this scene: rt_scene_release waits for these
+    // events only, not for the whole device (a server's steps on other scenes keep running).
+    std::vector<std::pair<hipStream_t, hipEvent_t>> last_use;
+
+    Scene() = default;
+    Scene(const Scene &) = delete;
+    Scene &operator=(const Scene &) = delete;
+    ~Scene() {
+        for (auto &u : last_use) (void)hipEventDestroy(u.second);
+    }
+    bool unmarked_use = false;  // used on a stream without an event of its own (see mark_use)
+
+    // Called after enqueueing work that reads the scene on `stream` (the context lock held).  Work
+    // being captured into a graph is not marked: its caller orders the graph's launches.  With
+    // may_create false (callers that must not allocate) a stream seen for the first time gets no
+    // event; the scene's release then waits for the whole device instead.
+    void mark_use(hipStream_t stream, bool may_create = true) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (stream && hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return;
+        hipEvent_t ev = nullptr;
+        for (auto &u : last_use)
+            if (u.first == stream) ev = u.second;
+        if (!ev && !may_create) {
+            unmarked_use = true;
+            return;
+        }
+        if (!ev) {
+            RT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            last_use.emplace_back(stream, ev);
+        }
+        RT_HIP(hipEventRecord(ev, stream));
+    }
+    // Blocks until every use has completed.
+    void wait_unused() const {
+        if (unmarked_use) RT_HIP(hipDeviceSynchronize());
+        for (const auto &u : last_use) RT_HIP(hipEventSynchronize(u.second));
+    }
     uint64_t device_bytes() const {
-        return wide.bytes + records.bytes + slot_prim.bytes + slot_mat.bytes + materials.bytes + binary.bytes;
+        return wide.bytes + records.bytes + slot_prim.bytes + slot_mat.bytes + materials.bytes + binary.bytes +
+               tris.bytes + spheres.bytes;
     }
 };
 
@@ -118,6 +158,7 @@ uint32_t split_group(const rt_context *ctx, uint32_t width, uint32_t height, uin
 constexpr uint32_t kRenderFlags = RT_RENDER_REFERENCE_MATERIALS | RT_RENDER_NO_PRIMARY_CACHE |
                                    RT_RENDER_SPLIT_SAMPLES | RT_RENDER_NO_SPLIT | RT_RENDER_WAVEFRONT |
                                    RT_RENDER_PERSISTENT_PATHS;
+constexpr uint32_t kTraceFlags = RT_TRACE_ANY | RT_TRACE_FP32;
 constexpr uint32_t kRenderChunk = 64;  // work items (one 8x8 tile) per atomic on the render work counter
 
 // Small host->device tables (batch item lists, render job lists) that a launch reads.
@@ -383,9 +424,13 @@ bool wave_schedule(uint32_t width, uint32_t height, uint32_t spp, uint32_t flags
 uint64_t per_pass_limit(uint64_t npix);  // samples of an npix-pixel image per pass
 void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, unsigned long long *segments,
                         const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes);
+// With `scratch` set, every buffer beyond the inputs and outputs comes from it (scratch_bytes long):
+// bounce-by-bounce renders run in as many passes as it needs (RT_ERR_BUDGET when one sample of the
+// image does not fit), and persistent launches whose per-sample storage does not fit run whole pixels
+// per lane (the same image).  Without it, storage comes from the context's scratch cache.
 void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height,
                    uint32_t spp, uint32_t flags, float *accum, unsigned long long *segments,
-                   const LaunchCfg &cfg, hipStream_t stream);
+                   const LaunchCfg &cfg, hipStream_t stream, char *scratch = nullptr, uint64_t scratch_bytes = 0);
 RenderJob make_render_job(const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,
                           uint32_t flags, float *accum);
 // One launch over all jobs (all scenes must have BVH width `bvh_width`); returns the work items launched.

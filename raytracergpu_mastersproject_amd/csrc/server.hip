@@ -134,6 +134,9 @@ struct rt_server {
 
     // rates measured from retired steps (work units per microsecond of device time)
     double sample_rate = 1000.0, ray_rate = 2000.0;
+    // host time (ms since creation) of the last admission of a request that is not a device-filling
+    // render: while such requests keep arriving, device-filling renders advance in short slices
+    double last_small_ms = -1e30;
     rt_server_stats stats{};
     rtgpu::Clock::time_point t0;
     std::thread worker;
@@ -181,6 +184,9 @@ void admit(rt_server *s) {
         r->offset = off;
         r->state = State::Active;
         r->st.admit_ms = s->now_ms();
+        if (r->req.kind != RT_REQ_RENDER ||
+            (uint64_t)r->req.width * r->req.height * r->req.spp < kWaveMinSamples)
+            s->last_small_ms = r->st.admit_ms;
         {
             std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
             if (r->req.kind == RT_REQ_RENDER) {
@@ -227,20 +233,29 @@ bool enqueue_step(rt_server *s) {
     // budget: its first members are not held long by its last.  Storage (path state, or
     // per-sample colours) comes from the scratch half of the arena and returns there when the
     // step retires.
+    //
+    // Small renders go first: a step carries the device-filling render only when no small render
+    // is waiting.  While small requests keep arriving (one admitted within the last two step
+    // budgets), a device-filling render advances in slices of an eighth of the budget, with at most
+    // one step in flight, so a small request that arrives behind it waits for one short slice.
     double budget = std::max(1.0, s->sample_rate * s->cfg.step_us);
     std::vector<std::pair<std::shared_ptr<Request>, uint32_t>> plan;
     uint64_t total = 0;
     auto npix_of = [](const Request &r) { return (uint64_t)r.req.width * r.req.height; };
     auto is_big = [&](const Request &r) { return npix_of(r) * r.remaining >= kWaveMinSamples; };
+    const bool small_recent = s->now_ms() - s->last_small_ms < 2.0 * s->cfg.step_us / 1000.0;
     bool big_step = false;
     if (!serial && !renders.empty()) {
-        const bool big = is_big(*renders[0]);
+        bool big = true;
+        for (auto &r : renders) big = big && is_big(*r);
         big_step = big;
+        if (big && small_recent && !s->inflight.empty() && queries.empty()) return false;
         std::vector<std::shared_ptr<Request>> kind;
         for (auto &r : renders)
             if (is_big(*r) == big && (!big || kind.empty())) kind.push_back(r);
         renders.swap(kind);
         if (!big) budget = std::max(1.0, budget / 5.0);
+        else if (small_recent) budget = std::max(1.0, budget / 8.0);
     }
     for (auto &r : renders) {
         const uint64_t npix = npix_of(*r);
@@ -341,7 +356,7 @@ bool enqueue_step(rt_server *s) {
         launch_render_jobs(s->ctx, kv.second, kv.first, s->segments.as<unsigned long long>(), rc, s->stream);
     if (!queries.empty()) {
         const double rshare = s->ray_rate * s->cfg.step_us / (double)queries.size();
-        std::vector<TraceItem> closest, any;
+        std::vector<TraceItem> groups[4];  // by trace flags: closest/any x refined/fp32
         for (auto &r : queries) {
             uint32_t q = serial ? r->remaining
                                 : (uint32_t)std::min<double>(r->remaining, std::max(65536.0, std::floor(rshare)));
@@ -349,18 +364,17 @@ bool enqueue_step(rt_server *s) {
             const uint32_t first = r->done_units;
             const rt_ray *rays = reinterpret_cast<const rt_ray *>(arena_ptr(s, r->offset)) + first;
             rt_hit *hits = reinterpret_cast<rt_hit *>(arena_ptr(s, r->offset + Arena::round(32ull * r->req.nrays))) + first;
-            ((r->req.trace_flags & RT_TRACE_ANY) ? any : closest).push_back(TraceItem{r->scene, rays, hits, q});
+            groups[r->req.trace_flags & kTraceFlags].push_back(TraceItem{r->scene, rays, hits, q});
             r->remaining -= q;
             r->done_units += q;
             r->st.steps++;
             step.rays += q;
             step.members.push_back(r);
         }
-        for (int pass = 0; pass < 2; ++pass) {
-            auto &items = pass ? any : closest;
+        for (uint32_t flags = 0; flags < 4; ++flags) {
+            auto &items = groups[flags];
             if (items.empty()) continue;
-            const uint32_t flags = pass ? RT_TRACE_ANY : RT_TRACE_CLOSEST;
-            LaunchCfg tc = resolve_cfg(s->ctx, *items[0].sc, &s->cfg.trace_cfg, pass != 0);
+            LaunchCfg tc = resolve_cfg(s->ctx, *items[0].sc, &s->cfg.trace_cfg, (flags & RT_TRACE_ANY) != 0);
             if (!s->cfg.trace_cfg.persistent) tc.persistent = 1;
             if (serial) launch_trace(s->ctx, *items[0].sc, items[0].rays, items[0].n, items[0].hits, nullptr, flags, tc,
                                      s->stream);
@@ -553,7 +567,7 @@ int rt_server_submit(rt_server *s, const rt_request *req, uint64_t *id) {
             RT_REQUIRE(req->spp > 0, "a render request needs at least one sample per pixel");
             r->remaining = req->spp;
         } else {
-            RT_REQUIRE((req->trace_flags & ~RT_TRACE_ANY) == 0, "unknown trace flags");
+            RT_REQUIRE((req->trace_flags & ~kTraceFlags) == 0, "unknown trace flags");
             RT_REQUIRE(req->nrays == 0 || req->rays, "rays is NULL");
             r->remaining = req->nrays;
         }

@@ -22,6 +22,8 @@ struct SceneView {
     const uint32_t *wide;
     const LeafRecord *rec;
     const uint32_t *slot_prim;
+    const rt_triangle *tris;  // world-space primitives, for the refined results
+    const rt_sphere *sph;
     uint32_t nwide;
     uint32_t _pad;
 };
@@ -87,6 +89,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
     const ChunkSchedule<uint32_t> sched(n, gridDim.x * (blockDim.x / 64u), true);
     Trav t;
     t.have = false;
+    t.has_pend = false;
     t.top = 0;
     for (;;) {
         const unsigned long long idle = __ballot(rid < 0);
@@ -126,6 +129,41 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
             rid = -1;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Refined results: t, u, v of every hit recomputed in double precision from the hit
+// primitive's stored fp32 geometry and the fp32 ray (refine_triangle / refine_sphere).
+// The primitive id, decided by the fp32 traversal, is untouched; so is a result whose
+// recomputation has no solution in double (a zero determinant, a negative discriminant).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void refine_one(const rt_triangle *tris, const rt_sphere *sph, const rt_ray *rays,
+                                           rt_hit *hits, uint32_t i) {
+    uint4 *hp = reinterpret_cast<uint4 *>(hits) + i;
+    uint4 h = *hp;
+    if (h.x == RT_NO_HIT) return;
+    v3 o, d;
+    float tmin, tmax;
+    load_ray(rays, i, o, tmin, d, tmax);
+    float t = __uint_as_float(h.y), u = __uint_as_float(h.z), v = __uint_as_float(h.w);
+    bool ok;
+    if (h.x & RT_SPHERE_BIT) {
+        const float4 c = *reinterpret_cast<const float4 *>(sph + (h.x & ~RT_SPHERE_BIT));
+        const float r = sph[h.x & ~RT_SPHERE_BIT].radius;
+        ok = refine_sphere(dvec(c.x, c.y, c.z), (double)r, dvec(o.x, o.y, o.z), dvec(d.x, d.y, d.z), t);
+    } else {
+        const float4 *tp = reinterpret_cast<const float4 *>(tris + h.x);
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        ok = refine_triangle(dvec(a.x, a.y, a.z), dvec(b.x, b.y, b.z), dvec(c.x, c.y, c.z), dvec(o.x, o.y, o.z),
+                             dvec(d.x, d.y, d.z), t, u, v);
+    }
+    if (ok) *hp = make_uint4(h.x, __float_as_uint(t), __float_as_uint(u), __float_as_uint(v));
+}
+
+__global__ __launch_bounds__(256) void k_refine_hits(const rt_ray *rays, rt_hit *hits, uint32_t n,
+                                                     const rt_triangle *tris, const rt_sphere *sph) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) refine_one(tris, sph, rays, hits, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -177,6 +215,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
     const ChunkSchedule<uint64_t> sched(a.total, gridDim.x * (blockDim.x / 64u), false);
     Trav t;
     t.have = false;
+    t.has_pend = false;
     t.top = 0;
     for (;;) {
         const unsigned long long idle = __ballot(!active);
@@ -224,12 +263,22 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
     }
 }
 
+// refined results of a fused launch: one thread per global ray index, item by binary search
+__global__ __launch_bounds__(256) void k_refine_batch(const BatchItemDev *items, uint32_t nitems, uint64_t total) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint32_t k = find_item(items, 0u, nitems, g);
+    const BatchItemDev &it = items[k];
+    refine_one(it.sc.tris, it.sc.sph, it.rays, it.hits, (uint32_t)(g - it.first));
+}
+
 __global__ void k_zero_counter(unsigned int *c) {
     if (threadIdx.x == 0) { c[0] = 0u; c[1] = 0u; }
 }
 
 SceneView view_of(const Scene &sc) {
-    return SceneView{sc.wide.as<uint32_t>(), sc.records.as<LeafRecord>(), sc.slot_prim.as<uint32_t>(), sc.nwide, 0u};
+    return SceneView{sc.wide.as<uint32_t>(), sc.records.as<LeafRecord>(), sc.slot_prim.as<uint32_t>(),
+                     sc.tris.as<rt_triangle>(), sc.spheres.as<rt_sphere>(), sc.nwide, 0u};
 }
 
 template <int W, int S> struct Kernels {
@@ -315,6 +364,8 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
         void *args[] = {&a};
         RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
     }
+    if (!(flags & RT_TRACE_FP32))
+        k_refine_hits<<<(uint32_t)(((uint64_t)n + 255u) / 256u), 256, 0, stream>>>(rays, hits, n, a.sc.tris, a.sc.sph);
     RT_HIP(hipGetLastError());
 }
 
@@ -373,6 +424,9 @@ void launch_trace_items(rt_context *ctx, const std::vector<TraceItem> &items, ui
             1, std::min<uint64_t>(persistent_grid(ctx, k, c.block, lds), (chunks + wpb - 1) / wpb));
         void *args[] = {&a};
         RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
+        if (!(flags & RT_TRACE_FP32))
+            k_refine_batch<<<(uint32_t)((total + 255u) / 256u), 256, 0, stream>>>(
+                static_cast<const BatchItemDev *>(table), (uint32_t)hv.size(), total);
         RT_HIP(hipGetLastError());
         ctx->tables.commit(slot, stream);
     }

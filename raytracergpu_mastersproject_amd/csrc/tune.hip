@@ -100,7 +100,7 @@ int rt_autotune(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t
     if (!ctx) return RT_ERR_INVALID;
     return guarded(ctx, [&] {
         RT_REQUIRE(n > 0, "autotune needs rays");
-        RT_REQUIRE((flags & ~RT_TRACE_ANY) == 0, "unknown trace flags");
+        RT_REQUIRE((flags & ~kTraceFlags) == 0, "unknown trace flags");
         Scene &s = ctx->scene(scene_id);
         const bool any = (flags & RT_TRACE_ANY) != 0;
         Timer timer;
@@ -126,7 +126,7 @@ int rt_autotune_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nite
                       rt_tune_result *out) {
     if (!ctx || (nitems && !items)) return RT_ERR_INVALID;
     return guarded(ctx, [&] {
-        RT_REQUIRE((flags & ~RT_TRACE_ANY) == 0, "unknown trace flags");
+        RT_REQUIRE((flags & ~kTraceFlags) == 0, "unknown trace flags");
         std::vector<TraceItem> v;
         uint32_t bound = 0;
         uint64_t total = 0;
@@ -159,7 +159,7 @@ int rt_autotune_build(rt_context *ctx, const rt_tune_scene *scenes, uint32_t nsc
     if (!ctx || (nscenes && !scenes)) return RT_ERR_INVALID;
     return guarded(ctx, [&] {
         RT_REQUIRE(nscenes > 0, "autotune needs sample scenes");
-        RT_REQUIRE((flags & ~RT_TRACE_ANY) == 0, "unknown trace flags");
+        RT_REQUIRE((flags & ~kTraceFlags) == 0, "unknown trace flags");
         uint64_t total = 0;
         for (uint32_t i = 0; i < nscenes; ++i) {
             RT_REQUIRE(scenes[i].nrays == 0 || (scenes[i].rays && scenes[i].hits), "sample scene without device rays");

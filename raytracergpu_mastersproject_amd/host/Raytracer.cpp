@@ -121,6 +121,41 @@ void Raytracer::mainLoop(uint32_t frames) {
     }
 }
 
+size_t Raytracer::sppSweep(const SppSweep &c, const std::string &csvPath) {
+    if (c.runsBeforeIncrease == 0) throw std::invalid_argument("sppSweep: runsBeforeIncrease must be positive");
+    if (c.increaseAmount == 0 && c.startRaysPerPixel <= c.maxRaysPerPixel)
+        throw std::invalid_argument("sppSweep: increaseAmount must be positive");
+    scene->setRaysPerPixel(c.startRaysPerPixel);
+    std::vector<std::vector<long long>> times;  // microseconds, per level
+    auto current = clock_t_::now();
+    for (uint32_t it = 0;; ++it) {
+        const auto now = clock_t_::now();
+        const long long frameUs = std::chrono::duration_cast<std::chrono::microseconds>(now - current).count();
+        current = now;
+        if (printTimings)
+            std::printf("Frame Time(us): %lld RaysPerPixel: %u Depth: %u\n", frameUs, scene->getRaysPerPixel(),
+                        scene->getMaxRaytraceDepth());
+        doIteration((float)frameUs);
+        if (it == 0) continue;
+        const size_t index = (it - 1) / c.runsBeforeIncrease;
+        if (scene->getRaysPerPixel() > c.maxRaysPerPixel) break;
+        if (times.size() == index) times.push_back({frameUs});
+        else times[index].push_back(frameUs);
+        if (it % c.runsBeforeIncrease == 0) scene->setRaysPerPixel(scene->getRaysPerPixel() + c.increaseAmount);
+    }
+    std::ofstream out(csvPath, std::ios::out | std::ios::trunc);
+    if (!out) throw std::runtime_error("cannot write " + csvPath);
+    size_t levels = 0;
+    for (size_t i = 0; i < times.size(); ++i) {
+        if (times[i].size() < c.runsBeforeIncrease) break;  // an unfinished level
+        long long sum = 0;
+        for (uint32_t j = 0; j < c.runsBeforeIncrease; ++j) sum += times[i][j];
+        out << (i + 1) << ", " << sum / (long long)c.runsBeforeIncrease << "us,\n";
+        ++levels;
+    }
+    return levels;
+}
+
 std::vector<float> Raytracer::accumulation() const {
     std::vector<float> out((size_t)width * height * 4);
     const_cast<rtgpu::Context &>(context).check(rt_util_memcpy_dtoh(out.data(), accum, out.size() * sizeof(float)),

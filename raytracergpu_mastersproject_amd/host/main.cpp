@@ -3,6 +3,7 @@
 //
 //   rtgpu_render [--scene complex|cornell|simple|mixed|spheres] [--size WxH] [--frames N]
 //                [--spp N] [--depth N] [--seed N] [--models DIR] [--out FILE.ppm] [--quiet]
+//                [--spp-sweep RUNS,START,MAX,STEP [--csv FILE]]   (Config::RunRayPerPixelIncreasingDemo)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,7 +20,9 @@ int main(int argc, char **argv) {
     uint32_t w = 800, h = 800, frames = 1, seed = 0;
     int spp = -1, depth = -1;
     bool quiet = false;
-    std::string dumpDir;
+    std::string dumpDir, csv = "runtimes.csv";
+    bool sweep = false;
+    RaytracerBVHRenderer::Raytracer::SppSweep sweepCfg;
     if (!std::getenv("RTGPU_MODEL_DIR")) {  // default: the models/ directory next to this executable
         char exe[4096];
         ssize_t n = readlink("/proc/self/exe", exe, sizeof exe - 1);
@@ -45,6 +48,13 @@ int main(int argc, char **argv) {
         else if (a == "--out") out = next();
         else if (a == "--quiet") quiet = true;
         else if (a == "--dump") dumpDir = next();
+        else if (a == "--spp-sweep") {
+            std::string s = next();
+            sweep = std::sscanf(s.c_str(), "%u,%u,%u,%u", &sweepCfg.runsBeforeIncrease, &sweepCfg.startRaysPerPixel,
+                                &sweepCfg.maxRaysPerPixel, &sweepCfg.increaseAmount) == 4;
+            if (!sweep) { std::fprintf(stderr, "--spp-sweep wants RUNS,START,MAX,STEP\n"); return 2; }
+        }
+        else if (a == "--csv") csv = next();
         else { std::fprintf(stderr, "unknown argument %s\n", a.c_str()); return 2; }
     }
     const std::map<std::string, RaytracerBVHRenderer::SceneFn> scenes = {
@@ -58,7 +68,12 @@ int main(int argc, char **argv) {
         if (depth >= 0) rt.getScene().setMaxRaytraceDepth((uint32_t)depth);
         rt.setRandomState(seed);
         rt.setPrintTimings(!quiet);
-        rt.mainLoop(frames);
+        if (sweep) {
+            const size_t levels = rt.sppSweep(sweepCfg, csv);
+            std::printf("{\"spp_sweep_levels\": %zu, \"csv\": \"%s\"}\n", levels, csv.c_str());
+        } else {
+            rt.mainLoop(frames);
+        }
         rt.writePPM(out);
         if (!dumpDir.empty()) {  // scene as flattened, the last frame's UBO and its accumulation image
             rtgpu_dump::write_scene(dumpDir, rt.getScene());

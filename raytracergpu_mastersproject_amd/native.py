@@ -30,6 +30,7 @@ BUILD_REFERENCE_MORTON = 2
 BUILD_KEEP_BINARY = 4
 TRACE_CLOSEST = 0
 TRACE_ANY = 1
+TRACE_FP32 = 2  # the reference's fp32 triangleHit t/u/v instead of the double-precision refinement
 RENDER_REFERENCE_MATERIALS = 1
 RENDER_NO_PRIMARY_CACHE = 2
 RENDER_SPLIT_SAMPLES = 4
@@ -293,6 +294,10 @@ class Event:
             pass
 
 
+def _trace_flags(any_hit: bool, fp32: bool) -> int:
+    return (TRACE_ANY if any_hit else TRACE_CLOSEST) | (TRACE_FP32 if fp32 else 0)
+
+
 def launch_config(block=0, lds_stack=0, persistent=1, refill=0):
     return LaunchConfig(block, lds_stack, persistent, refill)
 
@@ -366,14 +371,13 @@ class Context:
 
     # ---- ray queries ----
     def trace(self, sid, rays: DeviceArray, n: int, hits: DeviceArray, counts: DeviceArray | None = None,
-              any_hit=False, cfg: LaunchConfig | None = None, stream: Stream | None = None):
-        self._ck(lib.rt_trace(self.h, sid, _ptr(rays), n, _ptr(hits), _ptr(counts),
-                              TRACE_ANY if any_hit else TRACE_CLOSEST, C.byref(cfg) if cfg else None,
-                              stream.handle if stream else None))
+              any_hit=False, cfg: LaunchConfig | None = None, stream: Stream | None = None, fp32=False):
+        self._ck(lib.rt_trace(self.h, sid, _ptr(rays), n, _ptr(hits), _ptr(counts), _trace_flags(any_hit, fp32),
+                              C.byref(cfg) if cfg else None, stream.handle if stream else None))
 
-    def trace_batch(self, items, any_hit=False, cfg=None, stream=None):
+    def trace_batch(self, items, any_hit=False, cfg=None, stream=None, fp32=False):
         arr = (BatchItem * len(items))(*[BatchItem(s, n, _ptr(r), _ptr(h)) for (s, n, r, h) in items])
-        self._ck(lib.rt_trace_batch(self.h, arr, len(items), TRACE_ANY if any_hit else TRACE_CLOSEST,
+        self._ck(lib.rt_trace_batch(self.h, arr, len(items), _trace_flags(any_hit, fp32),
                                     C.byref(cfg) if cfg else None, stream.handle if stream else None))
 
     def autotune(self, sid, rays, n, hits, any_hit=False) -> TuneResult:
@@ -442,11 +446,11 @@ def render_request(scene_id, params, width, height, spp, flags=0, out: np.ndarra
     return r
 
 
-def trace_request(scene_id, rays: np.ndarray, any_hit=False, out: np.ndarray | None = None) -> Request:
+def trace_request(scene_id, rays: np.ndarray, any_hit=False, out: np.ndarray | None = None, fp32=False) -> Request:
     rays = np.ascontiguousarray(rays, dtype=L.RAY)
     r = Request()
     r.kind, r.scene_id, r.nrays = REQ_TRACE, scene_id, len(rays)
-    r.trace_flags = TRACE_ANY if any_hit else TRACE_CLOSEST
+    r.trace_flags = _trace_flags(any_hit, fp32)
     r.rays = rays.ctypes.data if len(rays) else None
     r._keep = rays  # host rays must live until admission
     if out is not None:

@@ -34,18 +34,38 @@ def upload_both(ctx, O, tris, spheres=None, mats=None, flags=0, width=4, leaf_si
     return sid, osc
 
 
-def gpu_trace(ctx, sid, rays, any_hit=False, counts=False, cfg=None):
+def gpu_trace(ctx, sid, rays, any_hit=False, counts=False, cfg=None, fp32=False):
     from raytracergpu_mastersproject_amd import native as N
     n = len(rays)
     dr = N.DeviceArray.from_numpy(rays)
     dh = N.DeviceArray(16 * max(n, 1))
     dc = N.DeviceArray(8 * max(n, 1)) if counts else None
-    ctx.trace(sid, dr, n, dh, dc, any_hit=any_hit, cfg=cfg)
+    ctx.trace(sid, dr, n, dh, dc, any_hit=any_hit, cfg=cfg, fp32=fp32)
     hits = dh.to_numpy(L.HIT, n)
     if counts:
         c = dc.to_numpy(np.uint32, 2 * n).reshape(n, 2)
         return hits, c
     return hits
+
+
+def fp64_errors(tris, rays, hits, idx):
+    """For hits[idx] (triangle hits): relative t error and absolute u, v error against a float64
+    Moller-Trumbore intersection of the hit triangle with the ray (the float64 CPU intersection
+    oracle of the parity bar)."""
+    tr = tris[hits["prim"][idx]]
+    v0, v1, v2 = (tr[k][:, :3].astype(np.float64) for k in ("v0", "v1", "v2"))
+    o, d = rays["o"][idx].astype(np.float64), rays["d"][idx].astype(np.float64)
+    e1, e2 = v1 - v0, v2 - v0
+    p = np.cross(d, e2)
+    det = np.einsum("ij,ij->i", e1, p)
+    s = o - v0
+    q = np.cross(s, e1)
+    u = np.einsum("ij,ij->i", s, p) / det
+    v = np.einsum("ij,ij->i", d, q) / det
+    t = np.einsum("ij,ij->i", e2, q) / det
+    t_err = np.abs(hits["t"][idx] - t) / np.maximum(np.abs(t), 1e-30)
+    uv_err = np.maximum(np.abs(hits["u"][idx] - u), np.abs(hits["v"][idx] - v))
+    return t_err, uv_err
 
 
 def compare_hits(g, o, check_uv=True):

@@ -39,3 +39,20 @@ def test_cpp_raytracer_frame_matches_oracle(native, tmp_path, scene, size, spp, 
     assert np.array_equal(accum.view(np.uint32), oacc.view(np.uint32))
     ppm = (tmp_path / "img.ppm").read_bytes()
     assert ppm.startswith(f"P6\n{w} {h}\n255\n".encode()) and len(ppm) == len(f"P6\n{w} {h}\n255\n") + 3 * w * h
+
+
+def test_spp_sweep_writes_runtimes_csv(native, tmp_path):
+    """Config::RunRayPerPixelIncreasingDemo (Config.hpp:17-23, RaytracerBVH.hpp:522-570): rays per pixel from 1,
+    +2 every 2 frames while at most 5, one mean frame time per level in runtimes.csv."""
+    csv = tmp_path / "runtimes.csv"
+    out = subprocess.run([str(PKG / "rtgpu_render"), "--scene", "simple", "--size", "32x32", "--quiet",
+                          "--spp-sweep", "2,1,5,2", "--csv", str(csv), "--out", str(tmp_path / "img.ppm")],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    meta = json.loads(out.stdout.strip().splitlines()[0])
+    lines = csv.read_text().splitlines()
+    assert meta["spp_sweep_levels"] == 3 and len(lines) == 3  # 1, 3 and 5 rays per pixel
+    for k, ln in enumerate(lines):
+        idx, t, rest = ln.split(",")
+        assert int(idx) == k + 1 and t.strip().endswith("us") and int(t.strip()[:-2]) > 0 and rest == ""
+    assert json.loads(out.stdout.strip().splitlines()[-1])["spp"] == 7  # the level past the maximum

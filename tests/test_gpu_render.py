@@ -160,9 +160,11 @@ def test_wavefront_equals_the_oracle(ctx, native, O, cache, monkeypatch):
     g, segs = gpu_render(native, ctx, sid, s["params"], w, h, spp, flags)
     assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
     assert segs == (osegs - w * h * (spp - 1) if cache else osegs)
-    # the late bounces run in one persistent launch from bounce 2 by default; from bounce 1,
-    # or never (every bounce its own launches), the image and the traced segments are the same
-    for finish in ("1", "99"):
+    # the late bounces run in one persistent launch from bounce 1 by default (bounce 0 is the
+    # shading pass over the primary hits); from bounce 2 or 3 (the bounces before it shaded by
+    # their own launches), or never (every bounce its own launches), the image and the traced
+    # segments are the same
+    for finish in ("2", "3", "99"):
         monkeypatch.setenv("RTGPU_WAVE_FINISH", finish)
         gf, segf = gpu_render(native, ctx, sid, s["params"], w, h, spp, flags)
         assert np.array_equal(gf.view(np.uint32), o.view(np.uint32)), finish

@@ -11,7 +11,7 @@ import pytest
 from raytracergpu_mastersproject_amd import layouts as L
 from raytracergpu_mastersproject_amd import workloads as W
 
-from helpers import compare_hits, gpu_trace, small_cluster_scene, upload_both, with_spheres
+from helpers import compare_hits, fp64_errors, gpu_trace, small_cluster_scene, upload_both, with_spheres
 
 pytestmark = pytest.mark.gpu
 
@@ -85,8 +85,34 @@ def test_spheres_and_triangles(ctx, native, O):
     rays = W.incoherent_rays(20000, lo, hi, seed=25)
     g = gpu_trace(ctx, sid, rays)
     ob = osc.trace(rays, O.BRUTE)
-    assert compare_hits(g, ob, check_uv=False) == 0
-    assert np.any((g["prim"] & L.SPHERE_BIT) != 0) and np.any(g["prim"] == ob["prim"])
+    assert compare_hits(g, ob) == 0
+    sphere = (g["prim"] != L.NO_HIT) & ((g["prim"] & L.SPHERE_BIT) != 0)
+    assert np.any(sphere) and np.any(g["prim"] == ob["prim"])
+    # a sphere hit's u, v are its centre's coordinates (raytraceBVH.comp:172-173)
+    for i in np.nonzero(sphere)[0][:50]:
+        assert (g["u"][i], g["v"][i]) == O.sphere_uv(sph["center"][int(g["prim"][i]) & 0x7FFFFFFF][:3])
+    g32 = gpu_trace(ctx, sid, rays, fp32=True)
+    assert compare_hits(g32, osc.trace(rays, O.BRUTE, fp32=True)) == 0
+    ctx.release(sid)
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_fp32_results_are_the_reference_arithmetic(ctx, native, O, any_hit):
+    """RT_TRACE_FP32: t, u, v exactly as the reference's fp32 triangleHit computes them; the default
+    refines them, keeping the ids; the refined values meet 1e-5 against float64 on every hit."""
+    tris, mats = small_cluster_scene(16, 4096, seed=17)
+    sid, osc = upload_both(ctx, O, tris, None, mats)
+    lo, hi = W.scene_bounds(tris)
+    rays = W.incoherent_rays(40000, lo, hi, seed=28)
+    g32, c32 = gpu_trace(ctx, sid, rays, any_hit=any_hit, counts=True, fp32=True)
+    g, c = gpu_trace(ctx, sid, rays, any_hit=any_hit, counts=True)
+    assert np.array_equal(g["prim"], g32["prim"]) and np.array_equal(c, c32)
+    assert compare_hits(g32, osc.trace(rays, O.WIDE, any_hit=any_hit, fp32=True)) == 0
+    assert compare_hits(g, osc.trace(rays, O.WIDE, any_hit=any_hit)) == 0
+    idx = np.nonzero(g["prim"] != L.NO_HIT)[0]
+    assert len(idx) > 1000
+    t_err, uv_err = fp64_errors(tris, rays, g, idx)
+    assert t_err.max() <= 1e-5 and uv_err.max() <= 1e-5
     ctx.release(sid)
 
 
