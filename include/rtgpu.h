@@ -222,6 +222,21 @@ typedef struct rt_render_job {
 int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, unsigned long long *segments,
                     const rt_launch_config *cfg, void *stream);
 
+/* rt_render without any device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, serving loops that forbid hipMalloc).  The
+ * image is bit-identical to rt_render's with the same arguments.  Every scratch buffer (per-sample
+ * storage, bounce-by-bounce path state and queues) comes from one static device array of
+ * RT_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, instead of the context's cache:
+ * bounce-by-bounce images run in as many passes as the array needs, and images that would keep
+ * per-sample storage keep whole pixels per lane when it does not fit.  RT_ERR_BUDGET when one
+ * sample of a bounce-by-bounce image (120 B per pixel) or a forced RT_RENDER_SPLIT_SAMPLES image
+ * (16 B per pixel-sample) exceeds the array.  Calls on one GPU share its array: calls that
+ * overlap on one GPU (other streams, other contexts of the process) race. */
+#define RT_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int rt_render_noalloc(rt_context *ctx, uint32_t scene_id, const rt_params *params, uint32_t width, uint32_t height,
+                      uint32_t spp, uint32_t flags, float *accum, unsigned long long *segments,
+                      const rt_launch_config *cfg, void *stream);
+
 /* ---------------- batched persistent launch (many scenes, one launch) ---------------- */
 
 typedef struct rt_batch_item {

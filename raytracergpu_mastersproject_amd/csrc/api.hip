@@ -55,6 +55,7 @@ int rt_create(rt_context **out, int device) {
         RT_HIP(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device));
         ctx->counters.alloc(16ull * rt_context::kCounterSlots);
         RT_HIP(hipMemset(ctx->counters.p, 0, ctx->counters.bytes));
+        ctx->noalloc_scratch = static_cast<char *>(noalloc_scratch_address());  // this device's copy
     });
     if (st != RT_OK) {
         delete ctx;
@@ -172,6 +173,21 @@ int rt_render(rt_context *ctx, uint32_t scene_id, const rt_params *params, uint3
         launch_render(ctx, s, *params, width, height, spp, flags, accum, segments, c,
                       static_cast<hipStream_t>(stream));
         s.mark_use(static_cast<hipStream_t>(stream));
+    });
+}
+
+int rt_render_noalloc(rt_context *ctx, uint32_t scene_id, const rt_params *params, uint32_t width, uint32_t height,
+                      uint32_t spp, uint32_t flags, float *accum, unsigned long long *segments,
+                      const rt_launch_config *cfg, void *stream) {
+    if (!ctx || !params) return RT_ERR_INVALID;
+    return guarded(ctx, [&] {
+        RT_REQUIRE((flags & ~kRenderFlags) == 0, "unknown render flags");
+        RT_REQUIRE(ctx->noalloc_scratch != nullptr, "the static scratch array has no device address");
+        Scene &s = ctx->scene(scene_id);
+        LaunchCfg c = resolve_cfg(ctx, s, cfg, false);
+        launch_render(ctx, s, *params, width, height, spp, flags, accum, segments, c, static_cast<hipStream_t>(stream),
+                      ctx->noalloc_scratch, noalloc_scratch_bytes());
+        s.mark_use(static_cast<hipStream_t>(stream), false);
     });
 }
 

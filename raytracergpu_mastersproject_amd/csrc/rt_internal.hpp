@@ -344,6 +344,8 @@ struct rt_context {
     // it enqueues a step, so a context may be shared by the caller and its servers.
     std::recursive_mutex mu;
     rtgpu::ScratchCache scratch;  // launch scratch (split samples, path storage)
+    // the device address of this GPU's static scratch array (rt_render_noalloc), taken at rt_create
+    char *noalloc_scratch = nullptr;
 
     rtgpu::Scene &scene(uint32_t id) {
         auto it = scenes.find(id);
@@ -431,6 +433,9 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
 void launch_render(rt_context *ctx, const Scene &sc, const rt_params &p, uint32_t width, uint32_t height,
                    uint32_t spp, uint32_t flags, float *accum, unsigned long long *segments,
                    const LaunchCfg &cfg, hipStream_t stream, char *scratch = nullptr, uint64_t scratch_bytes = 0);
+// noalloc.hip: the device address of the static scratch array on the current device (hipGetSymbolAddress)
+void *noalloc_scratch_address();
+uint64_t noalloc_scratch_bytes();
 RenderJob make_render_job(const Scene &sc, const rt_params &p, uint32_t width, uint32_t height, uint32_t spp,
                           uint32_t flags, float *accum);
 // One launch over all jobs (all scenes must have BVH width `bvh_width`); returns the work items launched.

@@ -125,6 +125,7 @@ _SIGS = {
     "rt_trace": (_i32, [_vp, _u32, _vp, _u32, _vp, _vp, _u32, C.POINTER(LaunchConfig), _vp]),
     "rt_accum_clear": (_i32, [_vp, _vp, _u32, _vp]),
     "rt_render": (_i32, [_vp, _u32, _vp, _u32, _u32, _u32, _u32, _vp, _vp, C.POINTER(LaunchConfig), _vp]),
+    "rt_render_noalloc": (_i32, [_vp, _u32, _vp, _u32, _u32, _u32, _u32, _vp, _vp, C.POINTER(LaunchConfig), _vp]),
     "rt_resolve": (_i32, [_vp, _vp, _u32, _u32, _vp, _vp]),
     "rt_trace_batch": (_i32, [_vp, C.POINTER(BatchItem), _u32, _u32, C.POINTER(LaunchConfig), _vp]),
     "rt_autotune": (_i32, [_vp, _u32, _vp, _u32, _vp, _u32, C.POINTER(TuneResult)]),
@@ -419,6 +420,14 @@ class Context:
         params = np.ascontiguousarray(params, dtype=L.PARAMS)
         self._ck(lib.rt_render(self.h, sid, params.ctypes.data, width, height, spp, flags, _ptr(accum),
                                _ptr(segments), C.byref(cfg) if cfg else None, stream.handle if stream else None))
+
+    def render_noalloc(self, sid, params: np.ndarray, width, height, spp, accum: DeviceArray, flags=0,
+                       segments: DeviceArray | None = None, cfg=None, stream=None):
+        """rt_render with every scratch buffer in the library's static device array (no allocation)."""
+        params = np.ascontiguousarray(params, dtype=L.PARAMS)
+        self._ck(lib.rt_render_noalloc(self.h, sid, params.ctypes.data, width, height, spp, flags, _ptr(accum),
+                                       _ptr(segments), C.byref(cfg) if cfg else None,
+                                       stream.handle if stream else None))
 
     def resolve(self, accum: DeviceArray, npix: int, spp: int, out: DeviceArray, stream=None):
         self._ck(lib.rt_resolve(self.h, _ptr(accum), npix, spp, _ptr(out), stream.handle if stream else None))
