@@ -697,7 +697,18 @@ static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float 
     const uint32_t W = sc->width;
     const uint32_t stride = 8u * W;
     const uint32_t *nodes = (const uint32_t *)sc->wide_nodes;
-    v3 idir = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    /* slab distances fma(plane, idir, n_lo or n_hi): n = -(o * idir) rounded once per ray, lowered for
+     * the planes an axis enters through and raised for those it leaves through by 4u|n|; a zero
+     * direction component takes idir = +-2^100 and no offset (traverse.hpp slab_axis) */
+    float idir[3], nlo[3], nhi[3];
+    const float dd[3] = {d.x, d.y, d.z}, oo[3] = {o.x, o.y, o.z};
+    for (int k = 0; k < 3; ++k) {
+        idir[k] = dd[k] != 0.0f ? 1.0f / dd[k] : copysignf(0x1p100f, dd[k]);
+        const float n = -(oo[k] * idir[k]);
+        const float e = dd[k] != 0.0f ? copysignf(0x1p-22f * fabsf(n), idir[k]) : 0.0f;
+        nlo[k] = n - e;
+        nhi[k] = n + e;
+    }
     float bt = tmax;
     uint32_t bslot = ORC_NO_HIT;
     wentry stack[1024];
@@ -757,9 +768,9 @@ static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float 
             uint32_t ch = nd[6u * W + c];
             if (ch == WIDE_EMPTY) continue;
             if (cnt) cnt->n_box++;
-            float x0 = (f[0 * W + c] - o.x) * idir.x, x1 = (f[1 * W + c] - o.x) * idir.x;
-            float y0 = (f[2 * W + c] - o.y) * idir.y, y1 = (f[3 * W + c] - o.y) * idir.y;
-            float z0 = (f[4 * W + c] - o.z) * idir.z, z1 = (f[5 * W + c] - o.z) * idir.z;
+            float x0 = fmaf(f[0 * W + c], idir[0], nlo[0]), x1 = fmaf(f[1 * W + c], idir[0], nhi[0]);
+            float y0 = fmaf(f[2 * W + c], idir[1], nlo[1]), y1 = fmaf(f[3 * W + c], idir[1], nhi[1]);
+            float z0 = fmaf(f[4 * W + c], idir[2], nlo[2]), z1 = fmaf(f[5 * W + c], idir[2], nhi[2]);
             float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
             float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), bt));
             if (tn <= tf * kRobust) {

@@ -111,8 +111,8 @@ template <> __device__ __forceinline__ void load_node<8>(const uint32_t *wide, u
 
 // order-preserving map float -> uint32 (total order, -0 < +0) and back
 __device__ __forceinline__ uint32_t ord_key(float f) {
-    uint32_t b = __float_as_uint(f);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    const uint32_t b = __float_as_uint(f);
+    return b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);  // negative: ~b; otherwise b | sign bit
 }
 __device__ __forceinline__ float ord_dec(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
@@ -123,6 +123,7 @@ constexpr uint32_t kOrdLowest = 0x00800000u;  // ord_key(-FLT_MAX): never culled
 // per-lane traversal state
 struct Trav {
     v3 o, d, idir;
+    v3 nlo, nhi;        // per-axis slab offsets of the low and high planes (see slab_axis)
     float tmin, bt;     // ray start; best t so far (initially tmax)
     uint32_t bslot;     // leaf slot of the best hit (kEmpty = none)
     float bu, bv;       // barycentrics of the best hit
@@ -189,14 +190,36 @@ template <int S> __device__ __forceinline__ void pop(Trav &t, const StackMem &m)
     t.have = v.x != kEmpty;
 }
 
-// A ray with a zero direction component that lies exactly in the plane of a box face gets
-// (face - origin) * inf = NaN on that axis and the box is culled, as the reference's
-// AABBhitCheck (raytraceBVH.comp:184-193) culls it: such rays miss primitives that a
-// brute-force test would report.
+// Slab distances take one FMA per plane: fma(plane, idir, n) with n = -(o * idir) rounded once
+// per ray.  That rounding is an absolute error of up to u|n| (u = 2^-24) in every distance of the
+// axis, which the relative widening kRobust does not cover, so each axis carries two offsets:
+// nlo for the planes it enters through and nhi for those it leaves through, n lowered and raised
+// by e = 4u|n| (which also covers the rounding of n -+ e).  Near distances come out no larger and
+// far distances no smaller than with the exact offset, and a box the exact test enters is never
+// culled.  The bound widens an axis only where that axis decides the entry or exit distance: a
+// per-ray widening by max e over the axes would be huge for a ray almost parallel to an axis
+// plane (|d_i| of 1e-7 gives |n| of 1e7) and would send it through most of the tree.
+// A zero direction component gets idir = +-2^100 instead of infinity, with e = 0: the products
+// with a power of two are exact, so that axis's distances are (plane - o) * 2^100 rounded once
+// (coordinates below 2^27 in magnitude), the slab of a direction component of 2^-100 with the
+// zero's sign.  A ray lying exactly in the plane of a box face on such an axis enters the box where
+// that nudged ray moves into it (low face with +0, high face with -0) and is culled on the other
+// face; the reference's AABBhitCheck (raytraceBVH.comp:184-193) culls it on both through 0 * inf =
+// NaN, and brute force finds what a cull misses.
+__device__ __forceinline__ void slab_axis(float d, float o, float &idir, float &nlo, float &nhi) {
+    idir = d != 0.0f ? 1.0f / d : copysignf(0x1p100f, d);
+    const float n = -(o * idir);
+    const float e = d != 0.0f ? copysignf(0x1p-22f * fabsf(n), idir) : 0.0f;
+    nlo = n - e;
+    nhi = n + e;
+}
+
 __device__ __forceinline__ void trav_init(Trav &t, v3 o, v3 d, float tmin, float tmax, uint32_t nwide) {
     t.o = o;
     t.d = d;
-    t.idir = v3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    slab_axis(d.x, o.x, t.idir.x, t.nlo.x, t.nhi.x);
+    slab_axis(d.y, o.y, t.idir.y, t.nlo.y, t.nhi.y);
+    slab_axis(d.z, o.z, t.idir.z, t.nlo.z, t.nhi.z);
     t.tmin = tmin;
     t.bt = tmax;
     t.bslot = kEmpty;
@@ -261,15 +284,16 @@ __device__ __forceinline__ void node_step(Trav &t, const StackMem &m, const uint
         for (int c = 0; c < W; ++c) {
             const bool valid = nd.child[c] != kEmpty;
             if constexpr (COUNT) t.nbox += valid ? 1u : 0u;
-            float x0 = (nd.lo[0][c] - t.o.x) * t.idir.x, x1 = (nd.hi[0][c] - t.o.x) * t.idir.x;
-            float y0 = (nd.lo[1][c] - t.o.y) * t.idir.y, y1 = (nd.hi[1][c] - t.o.y) * t.idir.y;
-            float z0 = (nd.lo[2][c] - t.o.z) * t.idir.z, z1 = (nd.hi[2][c] - t.o.z) * t.idir.z;
+            float x0 = fmaf(nd.lo[0][c], t.idir.x, t.nlo.x), x1 = fmaf(nd.hi[0][c], t.idir.x, t.nhi.x);
+            float y0 = fmaf(nd.lo[1][c], t.idir.y, t.nlo.y), y1 = fmaf(nd.hi[1][c], t.idir.y, t.nhi.y);
+            float z0 = fmaf(nd.lo[2][c], t.idir.z, t.nlo.z), z1 = fmaf(nd.hi[2][c], t.idir.z, t.nhi.z);
             float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), t.tmin));
             float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t.bt));
             const bool hit = valid && tn <= tf * kRobust;
             nh += hit ? 1 : 0;
             const uint32_t hi = ANY ? (uint32_t)c : ord_key(tn);
-            key[c] = hit ? (((uint64_t)hi << 32) | nd.child[c]) : ~0ull;
+            // a missed child's key sorts after every hit child's (ord_key of a distance is below ~0u)
+            key[c] = ((uint64_t)(hit ? hi : ~0u) << 32) | nd.child[c];
         }
         sort_keys<W>(key);
         if (nh == 0) {

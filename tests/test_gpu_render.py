@@ -2,9 +2,10 @@
 
 The soup image is compared with the oracle tracing every segment by brute force.  The other
 images are compared with the oracle replaying the GPU's own wide BVH (method WIDE), which is
-brute force except for rays lying exactly in a box-face plane: those are culled like the
-reference traversal culls them (pinned in test_oracle_bvh.py), and an odd image width puts the
-centre column's rays in the plane x = 0 of such scenes."""
+brute force except for rays with a zero direction component lying exactly in a box-face plane:
+the traversal takes the zero as 2^-100 with its sign and culls the box where that nudged ray moves
+out of it (pinned in test_oracle_bvh.py), and an odd image width puts the centre column's rays in
+the plane x = 0 of such scenes."""
 import numpy as np
 import pytest
 

@@ -191,6 +191,43 @@ def test_far_origins_grazing_rays_and_negative_tmin(ctx, native, O):
     ctx.release(sid)
 
 
+def test_rays_almost_parallel_to_an_axis_plane(ctx, native, O):
+    """Rays with one direction component of magnitude 1e-30 to 1e-3, in a scene 16 units from the
+    coordinate origin, against the same rays with that component exactly zero.
+
+    Each slab plane's offset carries a rounding bound of 4u|o_i/d_i|, which grows without limit as
+    d_i shrinks; it widens only the distances of its own axis, which grow as fast.  So these rays
+    visit about as many boxes as the zero-component rays (a per-ray widening by the largest bound
+    sent such rays through most of the tree).  Hits and per-ray counts equal the oracle's replay."""
+    tris, mats = small_cluster_scene(8, 2000, seed=29)
+    for k in ("v0", "v1", "v2"):
+        tris[k][:, :3] += 16.0
+    sid, osc = upload_both(ctx, O, tris, None, mats)
+    lo, hi = (np.asarray(x, np.float64) for x in W.scene_bounds(tris))
+    rng = W.SplitMix64(31)
+    n = 12000
+    rays = np.zeros(n, L.RAY)
+    rays["o"] = lo + (hi - lo) * rng.uniform(3 * n).reshape(n, 3)
+    d = rng.sphere_dirs(n)
+    axis = np.arange(n) % 3
+    tiny = np.exp(rng.uniform(n, np.log(1e-30), np.log(1e-3))) * np.where(rng.uniform(n) < 0.5, -1.0, 1.0)
+    d[np.arange(n), axis] = tiny
+    rays["d"] = d
+    rays["tmin"], rays["tmax"] = 0.001, 1e7
+    flat = rays.copy()
+    flat["d"][np.arange(n), axis] = 0.0
+    totals = []
+    for r in (rays, flat):
+        g, gc = gpu_trace(ctx, sid, r, counts=True)
+        ow, oc = osc.trace(r, O.WIDE, counts=True)
+        assert compare_hits(g, ow) == 0
+        assert np.array_equal(gc[:, 0], oc["n_box"]) and np.array_equal(gc[:, 1], oc["n_tri"])
+        assert _mismatch(g, osc.trace(r, O.BRUTE)).sum() <= n // 1000
+        totals.append(int(gc[:, 0].astype(np.int64).sum()))
+    assert totals[0] <= 1.25 * totals[1], totals
+    ctx.release(sid)
+
+
 def test_empty_scene_misses(ctx, native):
     sid = ctx.upload(None, None, W.make_materials([1, 1, 1], L.DIFFUSE))
     rays = W.incoherent_rays(1000, [-1, -1, -1], [1, 1, 1], seed=1)

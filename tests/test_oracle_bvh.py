@@ -213,11 +213,15 @@ def test_ties_choose_the_primitive_tested_last():
     assert np.all(slot_of[hb["prim"][hit]] < slot_of[twin])
 
 
-def test_rays_in_a_box_face_plane_are_culled_like_the_reference_traversal():
-    """A ray with a zero direction component lying exactly in the plane of a box face gives
-    (face - origin) * inf = NaN on that axis.  The reference's AABBhitCheck (raytraceBVH.comp:184-193,
-    restated with GLSL min/max) culls the box, and so does the product's traversal (replayed here
-    over a hand-built two-leaf wide node, IEEE minNum/maxNum slabs); brute force reports the hit."""
+def test_rays_in_a_box_face_plane():
+    """A ray with a zero direction component lying exactly in the plane of a box face.
+
+    The reference's AABBhitCheck (raytraceBVH.comp:184-193, restated with GLSL min/max) gets
+    (face - origin) * inf = NaN on that axis and culls the box on either face; brute force reports
+    the hit.  The wide traversal (replayed here over a hand-built two-leaf wide node) takes the zero
+    as 2^-100 with the zero's sign: the ray moves into the box from one face, out of it from the
+    other, so it enters the box on the low face with +0 and on the high face with -0, and is culled
+    on the other two, where the nudged ray only touches the box at t = 0."""
     t = np.zeros(2, O.TRIANGLE)
     t["v0"][0, :3], t["v1"][0, :3], t["v2"][0, :3] = [0, -1, 1], [1, -1, 1], [0, 1, 1]      # x in [0, 1], z = 1
     t["v0"][1, :3], t["v1"][1, :3], t["v2"][1, :3] = [2, -1, 4], [3, -1, 4], [3, 1, 4]      # x in [2, 3], z = 4
@@ -234,12 +238,17 @@ def test_rays_in_a_box_face_plane_are_culled_like_the_reference_traversal():
         node[6 * width + c] = (1 << 31) | c  # leaf: one primitive at slot c
     wide_scene = O.OracleScene(t, None, mats, None, np.array([0, 1], np.uint32), node, width)
     ref_scene = O.OracleScene(t, None, mats)  # the reference LBVH over the same triangles
-    r = np.zeros(4, O.RAY)
+    r = np.zeros(5, O.RAY)
     r["tmin"], r["tmax"] = 0.001, 1e7
-    r["o"] = [[0, 0, 0], [0, 0, 0], [3, 0.5, 0], [0.5, 0, 0]]
-    r["d"] = [[0, 0, 1], [-0.0, 0, 1], [0, 0, 1], [0, 0, 1]]  # faces x = 0 (both zero signs), x = 3; inside
+    # low face x = 0 of box 0 with both zero signs, high face x = 3 of box 1 with both; strictly inside
+    r["o"] = [[0, 0, 0], [0, 0, 0], [3, 0.5, 0], [3, 0.5, 0], [0.5, 0, 0]]
+    r["d"] = [[0, 0, 1], [-0.0, 0, 1], [0, 0, 1], [-0.0, 0, 1], [0, 0, 1]]
     brute = ref_scene.trace(r, O.BRUTE)
-    assert list(brute["prim"]) == [0, 0, 1, 0]
-    for hits in (ref_scene.trace(r, O.REF_BVH), wide_scene.trace(r, O.WIDE)):
-        assert list(hits["prim"][:3]) == [O.NO_HIT] * 3  # culled in the face plane
-        assert hits["prim"][3] == 0 and hits["t"][3] == brute["t"][3]  # strictly inside: found
+    assert list(brute["prim"]) == [0, 0, 1, 1, 0]
+    ref = ref_scene.trace(r, O.REF_BVH)
+    assert list(ref["prim"][:4]) == [O.NO_HIT] * 4  # culled in the face plane
+    wide = wide_scene.trace(r, O.WIDE)
+    assert list(wide["prim"][:4]) == [0, O.NO_HIT, O.NO_HIT, 1]  # entered where the nudged ray moves inward
+    assert np.array_equal(wide["t"][[0, 3]], brute["t"][[0, 3]])
+    for hits in (ref, wide):
+        assert hits["prim"][4] == 0 and hits["t"][4] == brute["t"][4]  # strictly inside: found
