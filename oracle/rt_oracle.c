@@ -714,31 +714,19 @@ static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float 
     wentry stack[1024];
     int sp = 0;
     if (sc->nwide == 0) return ORC_NO_HIT;
-    /* The kernel's per-lane state machine (traverse.hpp): a current entry, for closest hit one leaf
-     * set aside (`pend`) while node work goes on, and the stack.  A node step or a leaf step ends by
-     * popping when nothing is current, then setting aside a current leaf when no leaf is set aside
-     * yet (and popping again).  A leaf step tests the set-aside leaf, then a current leaf. */
-    wentry cur = {sc->wide_root, tmin}, pend = {0u, 0.0f};
-    int have = 1, has_pend = 0;
-#define ORC_POP_AND_STASH()                                                       \
+    /* The kernel's per-lane state machine (traverse.hpp): a current entry and the stack.  A node step
+     * or a leaf step ends by popping when nothing is current. */
+    wentry cur = {sc->wide_root, tmin};
+    int have = 1;
+#define ORC_POP()                                                                 \
     do {                                                                          \
         if (!have && sp > 0) { cur = stack[--sp]; have = 1; }                     \
-        if (!any && have && (cur.e & WIDE_LEAF) && !has_pend) {                   \
-            pend = cur;                                                           \
-            has_pend = 1;                                                         \
-            have = 0;                                                             \
-            if (sp > 0) { cur = stack[--sp]; have = 1; }                          \
-        }                                                                         \
     } while (0)
-    while (have || has_pend) {
-        if (!have || (cur.e & WIDE_LEAF)) { /* leaf step */
-            wentry lv[2];
-            int nl = 0;
-            if (has_pend) { lv[nl++] = pend; has_pend = 0; }
-            if (have) { lv[nl++] = cur; have = 0; }
-            for (int k = 0; k < nl; ++k) {
-                if (lv[k].tn > bt * kRobust) continue;
-                uint32_t first = lv[k].e & 0x07FFFFFFu, count = ((lv[k].e >> 27) & 15u) + 1u;
+    while (have) {
+        if (cur.e & WIDE_LEAF) { /* leaf step */
+            have = 0;
+            if (cur.tn <= bt * kRobust) {
+                uint32_t first = cur.e & 0x07FFFFFFu, count = ((cur.e >> 27) & 15u) + 1u;
                 for (uint32_t s = first; s < first + count; ++s) {
                     hitrec h;
                     if (cnt) cnt->n_tri++;
@@ -752,12 +740,12 @@ static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float 
                     }
                 }
             }
-            ORC_POP_AND_STASH();
+            ORC_POP();
             continue;
         }
         if (cur.tn > bt * kRobust) { /* node culled by a closer hit found since it was pushed */
             have = 0;
-            ORC_POP_AND_STASH();
+            ORC_POP();
             continue;
         }
         const uint32_t *nd = nodes + (size_t)stride * cur.e;
@@ -803,9 +791,9 @@ static uint32_t closest_wide(const orc_scene *sc, v3 o, v3 d, float tmin, float 
         if (nh > 0) cur = cand[0];
         else have = 0;
         if (sp > 1000) abort();
-        ORC_POP_AND_STASH();
+        ORC_POP();
     }
-#undef ORC_POP_AND_STASH
+#undef ORC_POP
     return bslot;
 }
 

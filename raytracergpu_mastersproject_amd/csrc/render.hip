@@ -348,7 +348,6 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
     ps.job = 0;
     Trav t;
     t.have = false;
-    t.has_pend = false;
     t.top = 0;
     uint32_t segs = 0;
     bool exhausted = false;
@@ -680,7 +679,6 @@ template <int W, int S> __global__ __launch_bounds__(256) void k_wave_finish(Fin
     v3 color{0.0f, 0.0f, 0.0f}, atten{1.0f, 1.0f, 1.0f};
     Trav t;
     t.have = false;
-    t.has_pend = false;
     t.top = 0;
     uint32_t segs = 0;  // continuation segments (the queue's own were counted when queued)
     bool exhausted = false;

@@ -89,7 +89,6 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
     const ChunkSchedule<uint32_t> sched(n, gridDim.x * (blockDim.x / 64u), true);
     Trav t;
     t.have = false;
-    t.has_pend = false;
     t.top = 0;
     for (;;) {
         const unsigned long long idle = __ballot(rid < 0);
@@ -215,7 +214,6 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
     const ChunkSchedule<uint64_t> sched(a.total, gridDim.x * (blockDim.x / 64u), false);
     Trav t;
     t.have = false;
-    t.has_pend = false;
     t.top = 0;
     for (;;) {
         const unsigned long long idle = __ballot(!active);

@@ -130,9 +130,6 @@ struct Trav {
     uint32_t cur;       // current entry (node index or leaf entry)
     float cur_tn;       // its entry distance
     bool have;          // cur is valid
-    bool has_pend;      // closest hit: a leaf set aside while the lane goes on with nodes (see stash)
-    uint32_t pend;      // that leaf entry
-    float pend_tn;      // and its entry distance
     uint32_t top, nspill;
     uint32_t nbox, ntri;
 };
@@ -228,19 +225,15 @@ __device__ __forceinline__ void trav_init(Trav &t, v3 o, v3 d, float tmin, float
     t.cur = 0u;  // root
     t.cur_tn = tmin;
     t.have = nwide > 0 && !has_nan(d);  // a NaN direction misses (the reference's root slab test fails)
-    t.has_pend = false;
     t.top = 0;
     t.nspill = 0;
     t.nbox = 0;
     t.ntri = 0;
 }
 
-__device__ __forceinline__ bool trav_done(const Trav &t) { return !t.have && !t.has_pend && t.top == 0; }
+__device__ __forceinline__ bool trav_done(const Trav &t) { return !t.have && t.top == 0; }
+__device__ __forceinline__ bool at_leaf(const Trav &t) { return t.have && (t.cur & kLeafBit); }
 __device__ __forceinline__ bool at_node(const Trav &t) { return t.have && !(t.cur & kLeafBit); }
-// no node to visit next: a leaf (set aside, current, or both) waits for the leaf phase
-__device__ __forceinline__ bool at_leaf(const Trav &t) {
-    return !at_node(t) && (t.has_pend || (t.have && (t.cur & kLeafBit)));
-}
 
 __device__ __forceinline__ void cswap64(uint64_t *k, int i, int j) {
     const uint64_t a = k[i], b = k[j];
@@ -265,8 +258,6 @@ template <int W> __device__ __forceinline__ void sort_keys(uint64_t *k) {
         cswap64(k, 1, 2); cswap64(k, 3, 4); cswap64(k, 5, 6);
     }
 }
-
-template <int S> __device__ __forceinline__ void stash(Trav &t, const StackMem &m);
 
 // Process the current wide node.  Precondition: at_node(t).
 //   closest: hit children in increasing (ord_key(tn), entry); ANY: in child order.
@@ -307,25 +298,6 @@ __device__ __forceinline__ void node_step(Trav &t, const StackMem &m, const uint
         }
     }
     if (!t.have && t.top > 0) pop<S>(t, m);
-    if constexpr (!ANY) stash<S>(t, m);
-}
-
-// Closest hit: a lane whose next entry is a leaf sets it aside (one at a time) and goes on with
-// the next entry of its stack, so it keeps doing node work while the wave is in a node phase
-// instead of idling until enough lanes wait on leaves.  It waits for a leaf phase only when its
-// next entry is a second leaf or its stack is empty; that phase tests both.  The set-aside leaf is
-// tested later than the reference's order would test it, against a best hit that can only have
-// shrunk meanwhile, which culls nothing it would have kept; the closest hit does not depend on the
-// order (the smallest t wins, ties to the smallest slot).  Each lane's sequence of steps still
-// depends only on its own ray, so per-ray counts stay reproducible.
-template <int S> __device__ __forceinline__ void stash(Trav &t, const StackMem &m) {
-    if (t.have && (t.cur & kLeafBit) && !t.has_pend) {
-        t.has_pend = true;
-        t.pend = t.cur;
-        t.pend_tn = t.cur_tn;
-        t.have = false;
-        if (t.top > 0) pop<S>(t, m);
-    }
 }
 
 // Test the primitives of one leaf entry; true when an any-hit query has found its hit.
@@ -355,25 +327,16 @@ __device__ __forceinline__ bool test_leaf(Trav &t, uint32_t entry, const LeafRec
     return false;
 }
 
-// Test the waiting leaves: the set-aside one, then the current entry when it is a leaf, each
-// unless a closer hit found since culls it.  Precondition: at_leaf(t).
+// Test the primitives of the current leaf, unless a closer hit found since it was pushed culls it.
+// Precondition: at_leaf(t).
 template <int S, bool ANY, bool COUNT = true>
 __device__ __forceinline__ void leaf_step(Trav &t, const StackMem &m, const LeafRecord *__restrict__ rec) {
-    if constexpr (!ANY) {
-        if (t.has_pend) {
-            if (t.pend_tn <= t.bt * kRobust) test_leaf<false, COUNT>(t, t.pend, rec);
-            t.has_pend = false;
-        }
+    if (t.cur_tn <= t.bt * kRobust && test_leaf<ANY, COUNT>(t, t.cur, rec)) {
+        t.top = 0;  // any hit: done
+        t.nspill = 0;
     }
-    if (t.have) {  // a leaf: at_leaf excludes a current node
-        if (t.cur_tn <= t.bt * kRobust && test_leaf<ANY, COUNT>(t, t.cur, rec)) {
-            t.top = 0;  // any hit: done
-            t.nspill = 0;
-        }
-        t.have = false;
-    }
+    t.have = false;
     if (t.top > 0) pop<S>(t, m);
-    if constexpr (!ANY) stash<S>(t, m);
 }
 
 // Wave-level choice of which lanes advance this iteration: leaves when enough lanes
