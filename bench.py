@@ -103,8 +103,8 @@ def traffic_table(fetch_csv: Path, write_csv: Path, configs, commands, out: Path
     cf = [v for k, v in fetch if k.startswith("__amd_rocclr_copyBuffer")][:5]
     cw = [v for k, v in write if k.startswith("__amd_rocclr_copyBuffer")][:5]
     f_scale, w_scale = copy_bytes / (np.mean(cf) * 1024), copy_bytes / (np.mean(cw) * 1024)
-    tf = [(k, v) for k, v in fetch if k.startswith("k_trace_")]
-    tw = [(k, v) for k, v in write if k.startswith("k_trace_")]
+    tf = [(k, v) for k, v in fetch if "::k_trace_" in k]
+    tw = [(k, v) for k, v in write if "::k_trace_" in k]
     assert len(tf) == len(tw) == len(configs), (len(tf), len(tw), len(configs))
     table = {"counters": "FETCH_SIZE, WRITE_SIZE (KiB per dispatch)", "fetch_scale": f_scale, "write_scale": w_scale,
              "calibration": f"{len(cf)} device-to-device copies of {copy_bytes} bytes", "commands": commands,
