@@ -98,7 +98,7 @@ int rt_scene_release(rt_context *ctx, uint32_t scene_id) {
         if (it->second->users)
             throw Error(RT_ERR_BUSY, "scene " + std::to_string(scene_id) + " is used by " +
                                          std::to_string(it->second->users) + " server request(s)");
-        it->second->wait_unused();  // work enqueued on any stream may still read the scene
+        ctx->wait_unused(*it->second);  // work enqueued on any stream may still read the scene
         ctx->scenes.erase(it);
     });
 }
@@ -153,7 +153,7 @@ int rt_trace(rt_context *ctx, uint32_t scene_id, const rt_ray *rays, uint32_t n,
         Scene &s = ctx->scene(scene_id);
         LaunchCfg c = resolve_cfg(ctx, s, cfg, (flags & RT_TRACE_ANY) != 0);
         launch_trace(ctx, s, rays, n, hits, counts, flags, c, static_cast<hipStream_t>(stream));
-        s.mark_use(static_cast<hipStream_t>(stream));
+        s.note_use(static_cast<hipStream_t>(stream), ctx->record_use(static_cast<hipStream_t>(stream)));
     });
 }
 
@@ -172,7 +172,7 @@ int rt_render(rt_context *ctx, uint32_t scene_id, const rt_params *params, uint3
         LaunchCfg c = resolve_cfg(ctx, s, cfg, false);
         launch_render(ctx, s, *params, width, height, spp, flags, accum, segments, c,
                       static_cast<hipStream_t>(stream));
-        s.mark_use(static_cast<hipStream_t>(stream));
+        s.note_use(static_cast<hipStream_t>(stream), ctx->record_use(static_cast<hipStream_t>(stream)));
     });
 }
 
@@ -187,7 +187,7 @@ int rt_render_noalloc(rt_context *ctx, uint32_t scene_id, const rt_params *param
         LaunchCfg c = resolve_cfg(ctx, s, cfg, false);
         launch_render(ctx, s, *params, width, height, spp, flags, accum, segments, c, static_cast<hipStream_t>(stream),
                       ctx->noalloc_scratch, noalloc_scratch_bytes());
-        s.mark_use(static_cast<hipStream_t>(stream), false);
+        s.note_use(static_cast<hipStream_t>(stream), ctx->record_use(static_cast<hipStream_t>(stream), false));
     });
 }
 
@@ -238,7 +238,8 @@ int rt_render_batch(rt_context *ctx, const rt_render_job *jobs, uint32_t njobs, 
         }
         for (auto &kv : by_width) launch_render_jobs(ctx, kv.second, kv.first, segments, c, st);
         for (auto &[s, rj] : waves) launch_render_wave(ctx, *s, rj, segments, c, st, nullptr, 0);
-        for (uint32_t i = 0; i < njobs; ++i) ctx->scene(jobs[i].scene_id).mark_use(st);
+        const int used = ctx->record_use(st);
+        for (uint32_t i = 0; i < njobs; ++i) ctx->scene(jobs[i].scene_id).note_use(st, used);
     });
 }
 
@@ -252,7 +253,8 @@ int rt_trace_batch(rt_context *ctx, const rt_batch_item *items, uint32_t nitems,
         if (nitems) c = resolve_cfg(ctx, ctx->scene(items[0].scene_id), cfg, any);
         if (!cfg && ctx->has_batch_tuned) c = any ? ctx->batch_any : ctx->batch_closest;
         launch_trace_batch(ctx, items, nitems, flags, c, static_cast<hipStream_t>(stream));
-        for (uint32_t i = 0; i < nitems; ++i) ctx->scene(items[i].scene_id).mark_use(static_cast<hipStream_t>(stream));
+        const int used = ctx->record_use(static_cast<hipStream_t>(stream));
+        for (uint32_t i = 0; i < nitems; ++i) ctx->scene(items[i].scene_id).note_use(static_cast<hipStream_t>(stream), used);
     });
 }
 

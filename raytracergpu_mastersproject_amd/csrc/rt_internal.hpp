@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -91,43 +92,20 @@ struct Scene {
     bool has_tuned = false;
     LaunchCfg tuned_closest, tuned_any;
     uint32_t users = 0;  // server requests holding this scene
-    // The last work enqueued on each stream that read this scene: rt_scene_release waits for these
-    // events only, not for the whole device (a server's steps on other scenes keep running).
-    std::vector<std::pair<hipStream_t, hipEvent_t>> last_use;
+    // Streams whose work read this scene.  rt_scene_release waits for the context's use event of
+    // each (recorded after every entry point's launches on that stream), not for the whole device:
+    // a server's steps on other scenes keep running.  The event may stand for later work on the
+    // same stream too, which only makes the wait longer.
+    std::vector<hipStream_t> used_streams;
+    bool unmarked_use = false;  // read by work that no use event covers: its release waits for the device
 
     Scene() = default;
     Scene(const Scene &) = delete;
     Scene &operator=(const Scene &) = delete;
-    ~Scene() {
-        for (auto &u : last_use) (void)hipEventDestroy(u.second);
-    }
-    bool unmarked_use = false;  // used on a stream without an event of its own (see mark_use)
 
-    // Called after enqueueing work that reads the scene on `stream` (the context lock held).  Work
-    // being captured into a graph is not marked: its caller orders the graph's launches.  With
-    // may_create false (callers that must not allocate) a stream seen for the first time gets no
-    // event; the scene's release then waits for the whole device instead.
-    void mark_use(hipStream_t stream, bool may_create = true) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (stream && hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return;
-        hipEvent_t ev = nullptr;
-        for (auto &u : last_use)
-            if (u.first == stream) ev = u.second;
-        if (!ev && !may_create) {
-            unmarked_use = true;
-            return;
-        }
-        if (!ev) {
-            RT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-            last_use.emplace_back(stream, ev);
-        }
-        RT_HIP(hipEventRecord(ev, stream));
-    }
-    // Blocks until every use has completed.
-    void wait_unused() const {
-        if (unmarked_use) RT_HIP(hipDeviceSynchronize());
-        for (const auto &u : last_use) RT_HIP(hipEventSynchronize(u.second));
-    }
+    // After an entry point's launches that read the scene on `stream`, with the result of
+    // rt_context::record_use for that stream.
+    void note_use(hipStream_t stream, int recorded);
     uint64_t device_bytes() const {
         return wide.bytes + records.bytes + slot_prim.bytes + slot_mat.bytes + materials.bytes + binary.bytes +
                tris.bytes + spheres.bytes;
@@ -169,7 +147,7 @@ constexpr uint32_t kRenderChunk = 64;  // work items (one 8x8 tile) per atomic o
 // waits on events recorded on the ring stream, which lives as long as the ring, never on
 // events of a caller's stream that may have been destroyed since.
 class TableRing {
-    static constexpr int kSlots = 32;
+    static constexpr int kSlots = 128;
     struct Slot {
         void *host = nullptr;
         DevBuf dev;
@@ -347,6 +325,44 @@ struct rt_context {
     // the device address of this GPU's static scratch array (rt_render_noalloc), taken at rt_create
     char *noalloc_scratch = nullptr;
 
+    // Per stream: an event recorded after the last entry point's launches on it (see record_use).
+    std::vector<std::pair<hipStream_t, hipEvent_t>> use_events;
+    static constexpr int kUseRecorded = 1, kUseCaptured = 0, kUseUnrecorded = -1;
+
+    rt_context() = default;
+    rt_context(const rt_context &) = delete;
+    rt_context &operator=(const rt_context &) = delete;
+    ~rt_context() {
+        for (auto &u : use_events) (void)hipEventDestroy(u.second);
+    }
+
+    // Called once after an entry point has enqueued its launches on `stream` (the context lock
+    // held).  kUseCaptured: the stream is being captured into a graph, whose caller orders the
+    // graph's launches.  kUseUnrecorded: the stream has no event yet and the caller may not
+    // create one (it must not allocate).
+    int record_use(hipStream_t stream, bool may_create = true) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (stream && hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+            return kUseCaptured;
+        hipEvent_t ev = nullptr;
+        for (auto &u : use_events)
+            if (u.first == stream) ev = u.second;
+        if (!ev && !may_create) return kUseUnrecorded;
+        if (!ev) {
+            RT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            use_events.emplace_back(stream, ev);
+        }
+        RT_HIP(hipEventRecord(ev, stream));
+        return kUseRecorded;
+    }
+    // Blocks until the work that read `sc` has completed.
+    void wait_unused(const rtgpu::Scene &sc) {
+        if (sc.unmarked_use) RT_HIP(hipDeviceSynchronize());
+        for (hipStream_t st : sc.used_streams)
+            for (auto &u : use_events)
+                if (u.first == st) RT_HIP(hipEventSynchronize(u.second));
+    }
+
     rtgpu::Scene &scene(uint32_t id) {
         auto it = scenes.find(id);
         if (it == scenes.end()) throw rtgpu::Error(RT_ERR_NOT_FOUND, "unknown scene id " + std::to_string(id));
@@ -359,6 +375,16 @@ struct rt_context {
 };
 
 namespace rtgpu {
+
+inline void Scene::note_use(hipStream_t stream, int recorded) {
+    if (recorded == rt_context::kUseCaptured) return;
+    if (recorded == rt_context::kUseUnrecorded) {
+        unmarked_use = true;
+        return;
+    }
+    if (std::find(used_streams.begin(), used_streams.end(), stream) == used_streams.end())
+        used_streams.push_back(stream);
+}
 
 // Runs f at the ABI boundary: holds the context lock, selects its device, and turns
 // exceptions into a status code with the message kept for rt_last_error.

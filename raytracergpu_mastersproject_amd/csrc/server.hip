@@ -3,15 +3,19 @@
 // Requests wait in a FIFO queue.  The head is admitted when a slot is free and the
 // request's buffers fit in the first half of the server's device arena (first-fit
 // allocation in one hipMalloc made at creation, so admission never calls the allocator).
-// Admitted requests advance in scheduling steps, enqueued on the server's stream up to
-// `pipeline` steps ahead of the device.  A step is one render launch plus one fused
-// ray-query launch over slices of every admitted query (launch_trace_items).  The render
-// launch carries either the oldest render alone, bounce by bounce as rt_render runs it,
-// when it has at least kWaveMinSamples pixel-samples left, or the admitted small renders
-// together in one persistent launch (launch_render_jobs, split layout), whole where they
-// fit the step's budget of device time (rates measured from earlier steps); renders
-// larger than the budget are cut into slices of samples.  A step's path state or
-// per-sample storage comes from the second half of the arena for the step's lifetime.
+// Admitted requests advance in scheduling steps on two lanes, each a stream with up to
+// `pipeline` steps enqueued ahead of the device:
+//   - the big lane carries renders of at least kWaveMinSamples pixel-samples (rt_render's
+//     threshold for filling the device), the oldest alone, bounce by bounce as rt_render runs
+//     it, in slices of the step's budget of device time (rates measured from earlier steps);
+//   - the small lane carries the other renders, together in one persistent launch
+//     (launch_render_jobs, split layout), whole where they fit the budget, plus one fused
+//     ray-query launch over slices of every admitted query (launch_trace_items).
+// The lanes' kernels share the device: a small step's kernels take compute units as the big
+// lane's bounce kernels release them, so a small request does not wait for a device-filling
+// step to end.  A request keeps its lane for life, so its steps stay in order on one stream.
+// A step's path state or per-sample storage comes from the second half of the arena for the
+// step's lifetime.
 // A request's last slice is followed by its resolve; when that step's event completes
 // the result is copied out and its arena range freed.
 //
@@ -21,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -96,6 +101,7 @@ struct Request {
     uint64_t bytes = 0, offset = 0;
     uint32_t remaining = 0;  // samples (render) or rays (query) not yet given to a step
     uint32_t done_units = 0;
+    bool big_lane = false;   // its steps run on the device-filling renders' stream (set at admission)
     rt_request_stats st{};
 };
 
@@ -105,6 +111,8 @@ struct Step {
     std::vector<std::shared_ptr<Request>> members;
     uint64_t samples = 0, rays = 0;
     uint64_t scratch_off = 0, scratch_bytes = 0;  // per-sample storage of the step's renders
+    bool big_lane = false;    // enqueued on the device-filling renders' stream
+    bool overlapped = false;  // shared the device with a step of the other stream
 };
 
 }  // namespace
@@ -118,7 +126,13 @@ struct rt_server {
     rtgpu::Arena scratch;  // second half: per-sample storage of the steps in flight
     uint64_t scratch_base = 0;
     rtgpu::DevBuf segments;  // u64 device counter of traced render segments
-    hipStream_t stream = nullptr;
+    // Two lanes: device-filling renders run on `stream`, everything else (small renders, ray
+    // queries) on `small_stream`, so a small request never waits behind a device-filling step:
+    // its kernels take compute units as the other lane's kernels release them.  The unbatched
+    // mode uses `stream` only.
+    hipStream_t stream = nullptr, small_stream = nullptr;
+    hipEvent_t t0_event = nullptr;  // recorded at creation: the origin of step intervals
+    std::deque<std::pair<float, float>> recent;  // device intervals of the last retired steps (busy_ms)
     std::vector<hipEvent_t> event_pool;
 
     std::mutex m;
@@ -135,8 +149,11 @@ struct rt_server {
     // rates measured from retired steps (work units per microsecond of device time)
     double sample_rate = 1000.0, ray_rate = 2000.0;
     // host time (ms since creation) of the last admission of a request that is not a device-filling
-    // render: while such requests keep arriving, device-filling renders advance in short slices
+    // render: while such requests keep arriving, a device-filling render that runs as one
+    // persistent launch (not bounce by bounce) advances in short slices
     double last_small_ms = -1e30;
+    double last_big_ms = -1e30;  // host time of the last big-lane step enqueued
+    double slice_div = 8.0;      // big-lane slices while small requests arrive: the budget / slice_div
     rt_server_stats stats{};
     rtgpu::Clock::time_point t0;
     std::thread worker;
@@ -173,6 +190,16 @@ hipEvent_t take_event(rt_server *s) {
 
 char *arena_ptr(rt_server *s, uint64_t off) { return static_cast<char *>(s->arena_mem.p) + off; }
 
+hipStream_t lane_stream(const rt_server *s, bool big_lane) {
+    return big_lane || (s->cfg.flags & RT_SERVER_SERIAL) ? s->stream : s->small_stream;
+}
+
+size_t lane_inflight(const rt_server *s, bool big_lane) {
+    size_t n = 0;
+    for (const Step &st : s->inflight) n += st.big_lane == big_lane ? 1 : 0;
+    return n;
+}
+
 // Admits queue heads while a slot and arena space are free.  Called with s->m held.
 void admit(rt_server *s) {
     const uint32_t slots = (s->cfg.flags & RT_SERVER_SERIAL) ? std::max(1u, s->cfg.pipeline) : s->cfg.max_active;
@@ -184,31 +211,33 @@ void admit(rt_server *s) {
         r->offset = off;
         r->state = State::Active;
         r->st.admit_ms = s->now_ms();
-        if (r->req.kind != RT_REQ_RENDER ||
-            (uint64_t)r->req.width * r->req.height * r->req.spp < kWaveMinSamples)
-            s->last_small_ms = r->st.admit_ms;
+        // A request keeps its lane for life, so its steps run in order on one stream.
+        r->big_lane = !(s->cfg.flags & RT_SERVER_SERIAL) && r->req.kind == RT_REQ_RENDER &&
+                      (uint64_t)r->req.width * r->req.height * r->req.spp >= kWaveMinSamples;
+        if (!r->big_lane) s->last_small_ms = r->st.admit_ms;
         {
             std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+            hipStream_t st = lane_stream(s, r->big_lane);
             if (r->req.kind == RT_REQ_RENDER) {
-                launch_accum_clear(reinterpret_cast<float *>(arena_ptr(s, off)), r->req.width * r->req.height,
-                                   s->stream);
+                launch_accum_clear(reinterpret_cast<float *>(arena_ptr(s, off)), r->req.width * r->req.height, st);
             } else if (r->req.nrays) {
-                RT_HIP(hipMemcpyAsync(arena_ptr(s, off), r->req.rays, 32ull * r->req.nrays, hipMemcpyHostToDevice,
-                                      s->stream));
+                RT_HIP(hipMemcpyAsync(arena_ptr(s, off), r->req.rays, 32ull * r->req.nrays, hipMemcpyHostToDevice, st));
             }
         }
         s->active.push_back(r);
     }
 }
 
-// Builds and enqueues one step.  Called with s->m held; returns false when nothing was scheduled.
-bool enqueue_step(rt_server *s) {
-    std::vector<std::shared_ptr<Request>> renders, queries;
-    for (auto &r : s->active)
-        if (r->state == State::Active && r->remaining > 0)
-            (r->req.kind == RT_REQ_RENDER ? renders : queries).push_back(r);
-    if (renders.empty() && queries.empty()) return false;
+// Builds and enqueues one step of a lane.  Called with s->m held; returns false when nothing was
+// scheduled.  The unbatched mode has one lane (big_lane true) that takes every request.
+bool enqueue_step(rt_server *s, bool big_lane) {
     const bool serial = (s->cfg.flags & RT_SERVER_SERIAL) != 0;
+    std::vector<std::shared_ptr<Request>> renders, queries;
+    for (auto &r : s->active) {
+        if (r->state != State::Active || r->remaining == 0 || (!serial && r->big_lane != big_lane)) continue;
+        (r->req.kind == RT_REQ_RENDER ? renders : queries).push_back(r);
+    }
+    if (renders.empty() && queries.empty()) return false;
     if (serial) {  // the oldest admitted request, whole
         std::shared_ptr<Request> first;
         for (auto &r : s->active)
@@ -217,46 +246,36 @@ bool enqueue_step(rt_server *s) {
         queries.clear();
         (first->req.kind == RT_REQ_RENDER ? renders : queries).push_back(first);
     }
+    const hipStream_t stream = lane_stream(s, big_lane);
     // Renders this step, in admission order: whole while they fit the step's sample budget;
     // the first that does not fit gets the number of samples that does (at least one when it
     // is the step's first render).
     //
-    // A render that fills the device by itself (at least kWaveMinSamples pixel-samples left,
-    // rt_render's own threshold) runs alone, bounce by bounce as rt_render would run it
-    // (below), so the small requests behind it are not held until the end of a launch that
-    // also carries it.  Small renders are fused
-    // into one persistent launch, every job in the split layout with one group size for the
-    // launch, chosen so the launch has ~32 work items per resident lane: its drain (lanes
-    // finishing their last items) stays short and each item traces its first segment once for
-    // its group of samples.  The oldest unfinished render decides which kind of step this is.
-    // A step of fused small renders finishes all of them together, so it gets a fifth of the
-    // budget: its first members are not held long by its last.  Storage (path state, or
-    // per-sample colours) comes from the scratch half of the arena and returns there when the
-    // step retires.
+    // Small lane: the admitted small renders are fused into one persistent launch, every job in
+    // the split layout with one group size for the launch, chosen so the launch has ~32 work
+    // items per resident lane: its drain (lanes finishing their last items) stays short and each
+    // item traces its first segment once for its group of samples.  Such a step finishes all its
+    // members together, so it gets a fifth of the budget: its first members are not held long by
+    // its last.  Ray queries ride along (slices of every admitted query).
     //
-    // Small renders go first: a step carries the device-filling render only when no small render
-    // is waiting.  While small requests keep arriving (one admitted within the last two step
-    // budgets), a device-filling render advances in slices of an eighth of the budget, with at most
-    // one step in flight, so a small request that arrives behind it waits for one short slice.
+    // Big lane: the oldest device-filling render (at least kWaveMinSamples pixel-samples at
+    // admission, rt_render's own threshold) runs alone, bounce by bounce as rt_render would run
+    // it (below), in slices of the budget.  Its last bounces run in one persistent launch that
+    // holds the compute units to its end, so while small requests keep arriving (one admitted
+    // within the last two step budgets) its slices are cut to budget / slice_div, and the worker
+    // keeps at most one of them in flight and none while a small step runs (worker_main).
+    //
+    // Storage (path state, or per-sample colours) comes from the scratch half of the arena and
+    // returns there when the step retires.
     double budget = std::max(1.0, s->sample_rate * s->cfg.step_us);
     std::vector<std::pair<std::shared_ptr<Request>, uint32_t>> plan;
     uint64_t total = 0;
     auto npix_of = [](const Request &r) { return (uint64_t)r.req.width * r.req.height; };
-    auto is_big = [&](const Request &r) { return npix_of(r) * r.remaining >= kWaveMinSamples; };
     const bool small_recent = s->now_ms() - s->last_small_ms < 2.0 * s->cfg.step_us / 1000.0;
-    bool big_step = false;
-    if (!serial && !renders.empty()) {
-        bool big = true;
-        for (auto &r : renders) big = big && is_big(*r);
-        big_step = big;
-        if (big && small_recent && !s->inflight.empty() && queries.empty()) return false;
-        std::vector<std::shared_ptr<Request>> kind;
-        for (auto &r : renders)
-            if (is_big(*r) == big && (!big || kind.empty())) kind.push_back(r);
-        renders.swap(kind);
-        if (!big) budget = std::max(1.0, budget / 5.0);
-        else if (small_recent) budget = std::max(1.0, budget / 8.0);
-    }
+    const bool big_step = !serial && big_lane && !renders.empty();
+    if (big_step) renders.resize(1);
+    else if (!serial && !renders.empty()) budget = std::max(1.0, budget / 5.0);
+    if (big_step && small_recent) budget = std::max(1.0, budget / s->slice_div);
     for (auto &r : renders) {
         const uint64_t npix = npix_of(*r);
         uint32_t q = r->remaining;
@@ -317,7 +336,7 @@ bool enqueue_step(rt_server *s) {
         step.scratch_bytes = wave ? scratch_bytes : 16ull * total;
     }
     std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
-    RT_HIP(hipEventRecord(step.start, s->stream));
+    RT_HIP(hipEventRecord(step.start, stream));
     const uint32_t group = (uint32_t)std::max<uint64_t>(1, total / (32ull * device_lanes(s)));
     char *storage = static_cast<char *>(s->arena_mem.p) + s->scratch_base + scratch_off;
     std::map<uint32_t, std::vector<RenderJob>> by_width;
@@ -335,10 +354,10 @@ bool enqueue_step(rt_server *s) {
         check_stack(*r->scene, rc);
         if (serial) {
             launch_render(s->ctx, *r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum,
-                          s->segments.as<unsigned long long>(), rc, s->stream);
+                          s->segments.as<unsigned long long>(), rc, stream);
         } else if (wave) {
             RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
-            launch_render_wave(s->ctx, *r->scene, j, s->segments.as<unsigned long long>(), rc, s->stream, storage,
+            launch_render_wave(s->ctx, *r->scene, j, s->segments.as<unsigned long long>(), rc, stream, storage,
                                scratch_bytes);
         } else {
             RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
@@ -353,7 +372,7 @@ bool enqueue_step(rt_server *s) {
         step.members.push_back(r);
     }
     for (auto &kv : by_width)
-        launch_render_jobs(s->ctx, kv.second, kv.first, s->segments.as<unsigned long long>(), rc, s->stream);
+        launch_render_jobs(s->ctx, kv.second, kv.first, s->segments.as<unsigned long long>(), rc, stream);
     if (!queries.empty()) {
         const double rshare = s->ray_rate * s->cfg.step_us / (double)queries.size();
         std::vector<TraceItem> groups[4];  // by trace flags: closest/any x refined/fp32
@@ -377,8 +396,8 @@ bool enqueue_step(rt_server *s) {
             LaunchCfg tc = resolve_cfg(s->ctx, *items[0].sc, &s->cfg.trace_cfg, (flags & RT_TRACE_ANY) != 0);
             if (!s->cfg.trace_cfg.persistent) tc.persistent = 1;
             if (serial) launch_trace(s->ctx, *items[0].sc, items[0].rays, items[0].n, items[0].hits, nullptr, flags, tc,
-                                     s->stream);
-            else launch_trace_items(s->ctx, items, flags, tc, s->stream);
+                                     stream);
+            else launch_trace_items(s->ctx, items, flags, tc, stream);
         }
     }
     for (auto &r : step.members) {
@@ -386,26 +405,53 @@ bool enqueue_step(rt_server *s) {
         r->state = State::Finishing;
         if (r->req.kind == RT_REQ_RENDER && !(r->req.render_flags & RT_REQ_OUT_ACCUM)) {
             float *accum = reinterpret_cast<float *>(arena_ptr(s, r->offset));
-            launch_resolve(accum, r->req.width * r->req.height, r->req.spp, accum, s->stream);  // in place
+            launch_resolve(accum, r->req.width * r->req.height, r->req.spp, accum, stream);  // in place
         }
         step.finishing.push_back(r);
     }
-    RT_HIP(hipEventRecord(step.end, s->stream));
+    RT_HIP(hipEventRecord(step.end, stream));
     s->stats.steps++;
+    step.big_lane = big_lane;
+    for (Step &o : s->inflight)
+        if (o.big_lane != step.big_lane) o.overlapped = step.overlapped = true;
     s->inflight.push_back(std::move(step));
     return true;
 }
 
-// Completes the oldest step (its end event has fired).  Called with s->m held.
-void retire_front(rt_server *s, double done_ms) {
-    Step step = std::move(s->inflight.front());
-    s->inflight.pop_front();
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, step.start, step.end) == hipSuccess && ms > 0.0f) {
-        s->stats.busy_ms += ms;
+// Device time covered by [a, b] and not by the recently retired steps' intervals: the steps of the
+// two lanes overlap, and busy_ms counts the time the device ran any step.
+double uncovered_ms(const std::deque<std::pair<float, float>> &recent, float a, float b) {
+    std::vector<std::pair<float, float>> cut;
+    for (const auto &iv : recent)
+        if (iv.second > a && iv.first < b) cut.emplace_back(std::max(iv.first, a), std::min(iv.second, b));
+    std::sort(cut.begin(), cut.end());
+    double covered = 0.0, upto = a;
+    for (const auto &iv : cut) {
+        if (iv.second <= upto) continue;
+        covered += iv.second - std::max<double>(iv.first, upto);
+        upto = iv.second;
+    }
+    return std::max(0.0, (double)(b - a) - covered);
+}
+
+// Completes inflight[i] (its end event has fired).  Called with s->m held.  A lane's steps retire
+// in order, so a request's last step retires after its others.
+void retire_step(rt_server *s, size_t i, double done_ms) {
+    Step step = std::move(s->inflight[i]);
+    s->inflight.erase(s->inflight.begin() + (std::ptrdiff_t)i);
+    float ms = 0.0f, a = 0.0f, b = 0.0f;
+    if (hipEventElapsedTime(&ms, step.start, step.end) == hipSuccess && ms > 0.0f &&
+        hipEventElapsedTime(&a, s->t0_event, step.start) == hipSuccess &&
+        hipEventElapsedTime(&b, s->t0_event, step.end) == hipSuccess) {
+        s->stats.busy_ms += uncovered_ms(s->recent, a, b);
+        s->recent.emplace_back(a, b);
+        if (s->recent.size() > 64) s->recent.pop_front();
+        // rates only from steps that had the device to themselves
         const double us = 1000.0 * ms;
-        if (step.samples && !step.rays) s->sample_rate = 0.5 * s->sample_rate + 0.5 * (double)step.samples / us;
-        if (step.rays && !step.samples) s->ray_rate = 0.5 * s->ray_rate + 0.5 * (double)step.rays / us;
+        if (!step.overlapped && step.samples && !step.rays)
+            s->sample_rate = 0.5 * s->sample_rate + 0.5 * (double)step.samples / us;
+        if (!step.overlapped && step.rays && !step.samples)
+            s->ray_rate = 0.5 * s->ray_rate + 0.5 * (double)step.rays / us;
     }
     s->stats.samples += step.samples;
     s->stats.rays += step.rays;
@@ -460,19 +506,45 @@ void worker_main(rt_server *s) {
     try {
         RT_HIP(hipSetDevice(s->ctx->device));
         for (;;) {
-            // retire steps that have finished
-            while (!s->inflight.empty() && hipEventQuery(s->inflight.front().end) == hipSuccess)
-                retire_front(s, s->now_ms());
+            // retire the steps that have finished, in either lane
+            for (size_t i = 0; i < s->inflight.size();) {
+                if (hipEventQuery(s->inflight[i].end) == hipSuccess) retire_step(s, i, s->now_ms());
+                else ++i;
+            }
             admit(s);
-            if (s->inflight.size() < s->cfg.pipeline && enqueue_step(s)) continue;
+            const bool serial = (s->cfg.flags & RT_SERVER_SERIAL) != 0;
+            bool progress = false;
+            if (serial) {
+                progress = s->inflight.size() < s->cfg.pipeline && enqueue_step(s, true);
+            } else {
+                if (lane_inflight(s, false) < s->cfg.pipeline && enqueue_step(s, false)) progress = true;
+                // While small requests keep arriving, the big lane runs one short slice at a time and
+                // waits while a small step runs, unless it has waited a whole step budget already.
+                const double now = s->now_ms();
+                const bool small_recent = now - s->last_small_ms < 2.0 * s->cfg.step_us / 1000.0;
+                const bool held = small_recent && lane_inflight(s, false) > 0 &&
+                                  now - s->last_big_ms < s->cfg.step_us / 1000.0;
+                if (!held && lane_inflight(s, true) < (small_recent ? 1u : s->cfg.pipeline) && enqueue_step(s, true)) {
+                    s->last_big_ms = now;
+                    progress = true;
+                }
+            }
+            if (progress) continue;
             if (!s->inflight.empty()) {
-                hipEvent_t e = s->inflight.front().end;
-                lk.unlock();
-                const hipError_t err = hipEventSynchronize(e);
-                const double t = s->now_ms();
-                lk.lock();
-                RT_HIP(err);
-                retire_front(s, t);
+                // Wait for the oldest small-lane step when one is in flight (it is short).  With only
+                // device-filling steps in flight, poll instead, so a request submitted meanwhile
+                // starts at once on the small lane rather than after the device-filling step.
+                size_t w = 0;
+                while (w + 1 < s->inflight.size() && s->inflight[w].big_lane && !serial) ++w;
+                if (serial || !s->inflight[w].big_lane) {
+                    hipEvent_t e = s->inflight[w].end;
+                    lk.unlock();
+                    const hipError_t err = hipEventSynchronize(e);
+                    lk.lock();
+                    RT_HIP(err);
+                } else {
+                    s->cv_work.wait_for(lk, std::chrono::microseconds(200));
+                }
                 continue;
             }
             if (s->stop && s->queue.empty() && s->active.empty()) break;
@@ -485,6 +557,14 @@ void worker_main(rt_server *s) {
     } catch (const std::exception &e) {
         fail_all(s, RT_ERR_HIP, e.what());
     }
+}
+
+void destroy_streams(rt_server *s) {
+    if (s->t0_event) (void)hipEventDestroy(s->t0_event);
+    if (s->small_stream) (void)hipStreamDestroy(s->small_stream);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->t0_event = nullptr;
+    s->small_stream = s->stream = nullptr;
 }
 
 }  // namespace
@@ -518,16 +598,20 @@ int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **o
         RT_HIP(hipMemset(s->segments.p, 0, 8));
         // non-blocking: the server's work does not serialise with the legacy default stream
         RT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        RT_HIP(hipStreamCreateWithFlags(&s->small_stream, hipStreamNonBlocking));
+        if (const char *v = std::getenv("RTGPU_SERVER_SLICE_DIV")) s->slice_div = std::max(1.0, std::atof(v));
+        RT_HIP(hipEventCreate(&s->t0_event));
+        RT_HIP(hipEventRecord(s->t0_event, s->stream));
         s->stats.arena_bytes = s->arena.size();
         s->t0 = Clock::now();
         s->worker = std::thread(worker_main, s.get());
     } catch (const Error &e) {
         ctx->last_error = e.what();
-        if (s && s->stream) (void)hipStreamDestroy(s->stream);
+        if (s) destroy_streams(s.get());
         return e.status;
     } catch (const std::exception &e) {
         ctx->last_error = e.what();
-        if (s && s->stream) (void)hipStreamDestroy(s->stream);
+        if (s) destroy_streams(s.get());
         return RT_ERR_INVALID;
     }
     *out = s.release();
@@ -543,9 +627,10 @@ int rt_server_destroy(rt_server *s) {
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
     const int st = s->fatal;
-    (void)hipStreamSynchronize(s->stream);
+    for (hipStream_t st : {s->stream, s->small_stream})
+        if (st) (void)hipStreamSynchronize(st);
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(s->stream);
+    destroy_streams(s);
     delete s;
     return st;
 }

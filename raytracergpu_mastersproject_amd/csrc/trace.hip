@@ -261,11 +261,18 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
     }
 }
 
-// refined results of a fused launch: one thread per global ray index, item by binary search
+// refined results of a fused launch: one thread per global ray index.  One binary search per block
+// finds the item of the block's first ray; a thread then steps over the items that start within the
+// block (few: an item holds whole batches of rays).
 __global__ __launch_bounds__(256) void k_refine_batch(const BatchItemDev *items, uint32_t nitems, uint64_t total) {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t first_item;
+    const uint64_t base = (uint64_t)blockIdx.x * blockDim.x;
+    if (threadIdx.x == 0) first_item = find_item(items, 0u, nitems, base);
+    __syncthreads();
+    const uint64_t g = base + threadIdx.x;
     if (g >= total) return;
-    const uint32_t k = find_item(items, 0u, nitems, g);
+    uint32_t k = first_item;
+    while (k + 1u < nitems && items[k + 1u].first <= g) ++k;
     const BatchItemDev &it = items[k];
     refine_one(it.sc.tris, it.sc.sph, it.rays, it.hits, (uint32_t)(g - it.first));
 }
