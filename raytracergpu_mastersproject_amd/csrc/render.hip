@@ -936,7 +936,8 @@ uint32_t wave_finish_bounce() {
 // paths, and no more than `scratch` holds when the caller provides it (else the storage comes
 // from the context's scratch cache).
 void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, unsigned long long *segments,
-                        const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes) {
+                        const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes,
+                        rt_hit *primary, bool primary_ready) {
     const uint32_t npix = j.width * j.height;
     uint64_t per_pass = std::min<uint64_t>(j.spp, per_pass_limit(npix));
     if (scratch) {
@@ -959,6 +960,7 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
     ids[1] = ids[0] + maxp;
     uint32_t *count = ids[1] + maxp;  // two queue counters and a work counter
     const bool cache = !(j.flags & RT_RENDER_NO_PRIMARY_CACHE);
+    if (!cache) primary = nullptr;
     const uint32_t block = kShadeBlock;
     // the shading kernels wait on scattered path state and leaf records: many waves per CU
     const uint32_t full = (uint32_t)ctx->num_cus * 8u;
@@ -974,18 +976,20 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
         a.att = att;
         a.segments = segments;
         const auto grid_for = [&](uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + block - 1) / block); };
-        if (j.depth > 0) {
+        rt_hit *first_hits = primary ? primary : hits;
+        if (j.depth > 0 && !(primary && primary_ready)) {
             const uint64_t nprim = cache ? npix : paths;
             a.rays_out = rays[0];
             k_wave_primary<<<grid_for(nprim), block, 0, stream>>>(a);
             RT_HIP(hipGetLastError());
-            launch_trace_queue(ctx, sc, rays[0], (uint32_t)nprim, nullptr, hits, tc, stream);
+            launch_trace_queue(ctx, sc, rays[0], (uint32_t)nprim, nullptr, first_hits, tc, stream);
+            primary_ready = primary != nullptr;
         }
         int cur = 0;
         const uint32_t finish_at = wave_finish_bounce();
         k_zero<<<1, 64, 0, stream>>>(count + cur);
         a.bounce = 0;
-        a.hits_in = hits;
+        a.hits_in = first_hits;
         a.rays_out = rays[cur];
         a.ids_out = ids[cur];
         a.n_out = count + cur;

@@ -450,8 +450,12 @@ constexpr uint64_t kWaveMinSamples = 1ull << 22;
 // rt_render's choice of schedule: bounce by bounce (true) or one persistent launch
 bool wave_schedule(uint32_t width, uint32_t height, uint32_t spp, uint32_t flags);
 uint64_t per_pass_limit(uint64_t npix);  // samples of an npix-pixel image per pass
+// With `primary` (npix hits, caller-owned), the pixels' primary hits are kept there: traced by this call
+// unless `primary_ready`, then read by every pass.  A render cut into several calls traces them once
+// (camera rays carry no jitter, so every sample of a pixel starts with the same hit).
 void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, unsigned long long *segments,
-                        const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes);
+                        const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes,
+                        rt_hit *primary = nullptr, bool primary_ready = false);
 // With `scratch` set, every buffer beyond the inputs and outputs comes from it (scratch_bytes long):
 // bounce-by-bounce renders run in as many passes as it needs (RT_ERR_BUDGET when one sample of the
 // image does not fit), and persistent launches whose per-sample storage does not fit run whole pixels

@@ -102,6 +102,7 @@ struct Request {
     uint32_t remaining = 0;  // samples (render) or rays (query) not yet given to a step
     uint32_t done_units = 0;
     bool big_lane = false;   // its steps run on the device-filling renders' stream (set at admission)
+    bool primary_ready = false;  // a big-lane render's primary hits have been traced (see request_bytes)
     rt_request_stats st{};
 };
 
@@ -166,12 +167,18 @@ struct rt_server {
 namespace rtgpu {
 namespace {
 
-// A render holds its accumulation image plus, when its slices run in the split layout, the
-// per-(pixel, sample) storage of one slice (at most max_spp_step samples).  The unbatched
-// server renders whole requests through rt_render's own scratch.  A ray query holds rays + hits.
-uint64_t request_bytes(const rt_request &r) {
+// A request's own arena range: a render's accumulation image, and for a device-filling render of
+// the batched server its pixels' primary hits after it (16 B each, traced once for all its
+// slices); a ray query's rays and hits.  Step storage comes from the arena's other half, and the
+// unbatched server renders whole requests through rt_render's own scratch.
+bool big_render(const rt_request &r, bool serial) {
+    return !serial && r.kind == RT_REQ_RENDER && (uint64_t)r.width * r.height * r.spp >= kWaveMinSamples;
+}
+
+uint64_t request_bytes(const rt_request &r, bool serial) {
     if (r.kind != RT_REQ_RENDER) return Arena::round(32ull * r.nrays) + Arena::round(16ull * r.nrays);
-    return Arena::round(16ull * r.width * r.height);
+    const uint64_t image = Arena::round(16ull * r.width * r.height);
+    return big_render(r, serial) ? 2 * image : image;
 }
 
 // resident lanes of the device, about 1024 per CU
@@ -212,8 +219,7 @@ void admit(rt_server *s) {
         r->state = State::Active;
         r->st.admit_ms = s->now_ms();
         // A request keeps its lane for life, so its steps run in order on one stream.
-        r->big_lane = !(s->cfg.flags & RT_SERVER_SERIAL) && r->req.kind == RT_REQ_RENDER &&
-                      (uint64_t)r->req.width * r->req.height * r->req.spp >= kWaveMinSamples;
+        r->big_lane = big_render(r->req, (s->cfg.flags & RT_SERVER_SERIAL) != 0);
         if (!r->big_lane) s->last_small_ms = r->st.admit_ms;
         {
             std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
@@ -357,8 +363,11 @@ bool enqueue_step(rt_server *s, bool big_lane) {
                           s->segments.as<unsigned long long>(), rc, stream);
         } else if (wave) {
             RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
+            // the primary hits, kept after the image, are traced by the render's first bounce-by-bounce slice
+            rt_hit *primary = reinterpret_cast<rt_hit *>(arena_ptr(s, r->offset + Arena::round(16ull * npix)));
             launch_render_wave(s->ctx, *r->scene, j, s->segments.as<unsigned long long>(), rc, stream, storage,
-                               scratch_bytes);
+                               scratch_bytes, r->big_lane ? primary : nullptr, r->primary_ready);
+            r->primary_ready = r->big_lane;
         } else {
             RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
             make_split(j, reinterpret_cast<float4 *>(storage), std::min(group, q));
@@ -543,7 +552,7 @@ void worker_main(rt_server *s) {
                     lk.lock();
                     RT_HIP(err);
                 } else {
-                    s->cv_work.wait_for(lk, std::chrono::microseconds(200));
+                    s->cv_work.wait_for(lk, std::chrono::microseconds(50));
                 }
                 continue;
             }
@@ -663,7 +672,7 @@ int rt_server_submit(rt_server *s, const rt_request *req, uint64_t *id) {
             check_stack(*r->scene, resolve_cfg(s->ctx, *r->scene, render ? &s->cfg.render_cfg : &s->cfg.trace_cfg,
                                                !render && (req->trace_flags & RT_TRACE_ANY)));
         }
-        r->bytes = request_bytes(*req);
+        r->bytes = request_bytes(*req, (s->cfg.flags & RT_SERVER_SERIAL) != 0);
     } catch (const Error &e) {
         std::lock_guard<std::mutex> lk(s->m);
         s->last_error = e.what();
