@@ -328,9 +328,15 @@ def test_big_request_runs_bounce_by_bounce(ctx, native, scenes):
         assert np.array_equal(out.reshape(-1).view(np.uint32), expect.view(np.uint32))
 
 
-def test_big_and_small_requests_mixed(ctx, native, scenes):
-    """Two device-filling renders (bounce-by-bounce steps) submitted among small renders and ray
-    queries (fused steps): every result equals the request run alone."""
+@pytest.mark.parametrize("slice_div", [None, "1", "32"])
+def test_big_and_small_requests_mixed(ctx, native, scenes, monkeypatch, slice_div):
+    """Two device-filling renders (bounce-by-bounce steps on the big lane) submitted among small
+    renders and ray queries (fused steps on the small lane): every result equals the request run
+    alone.  The big lane's slice while small requests arrive is the default eighth of the step
+    budget, the whole budget, or a 32nd (many slices, each reading the render's cached primary
+    hits)."""
+    if slice_div is not None:
+        monkeypatch.setenv("RTGPU_SERVER_SLICE_DIV", slice_div)
     ids, params, soup, ico = scenes
     reqs, expect = _mixed_requests(native, ids, params, _rays(ids, (soup, ico)))
     for name, w, h, spp, rs in (("ico", 512, 512, 36, 31), ("soup4", 640, 480, 30, 32)):
@@ -344,4 +350,6 @@ def test_big_and_small_requests_mixed(ctx, native, scenes):
     st = srv.stats()
     srv.close()
     assert all(s.status == 0 for s in stats) and st.completed == len(reqs)
+    if slice_div == "32":
+        assert max(s.steps for s in stats) > 4  # the big renders went in many slices
     _check_results(native, ctx, ids, expect)
