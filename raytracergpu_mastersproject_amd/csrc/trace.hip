@@ -75,7 +75,34 @@ __global__ __launch_bounds__(256) void k_trace_direct(TraceArgs a) {
     store_hit(a.sc, t, a.hits, a.counts, i);
 }
 
-template <int W, int S, bool ANY, bool COUNT>
+// The stored result of a finished ray, with t, u, v of a hit refined in double as k_refine_hits
+// refines them (the same operations on the same fp32 inputs, so the same bits).
+__device__ __forceinline__ void store_refined(const SceneView &sc, const Trav &t, rt_hit *hits, uint32_t *counts,
+                                              uint32_t i) {
+    uint4 h;
+    if (t.bslot == kEmpty) {
+        h = make_uint4(RT_NO_HIT, __float_as_uint(t.bt), 0u, 0u);
+    } else {
+        const uint32_t prim = sc.slot_prim[t.bslot];
+        float tt = t.bt, u = t.bu, v = t.bv;
+        if (prim & RT_SPHERE_BIT) {
+            const float4 c = *reinterpret_cast<const float4 *>(sc.sph + (prim & ~RT_SPHERE_BIT));
+            const float r = sc.sph[prim & ~RT_SPHERE_BIT].radius;
+            refine_sphere(dvec(c.x, c.y, c.z), (double)r, dvec(t.o.x, t.o.y, t.o.z), dvec(t.d.x, t.d.y, t.d.z), tt);
+        } else {
+            const float4 *tp = reinterpret_cast<const float4 *>(sc.tris + prim);
+            const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+            refine_triangle(dvec(p0.x, p0.y, p0.z), dvec(p1.x, p1.y, p1.z), dvec(p2.x, p2.y, p2.z),
+                            dvec(t.o.x, t.o.y, t.o.z), dvec(t.d.x, t.d.y, t.d.z), tt, u, v);
+        }
+        h = make_uint4(prim, __float_as_uint(tt), __float_as_uint(u), __float_as_uint(v));
+    }
+    reinterpret_cast<uint4 *>(hits)[i] = h;
+    if (counts) reinterpret_cast<uint2 *>(counts)[i] = make_uint2(t.nbox, t.ntri);
+}
+
+// REFINE: store each hit with t, u, v refined in double (store_refined) instead of the fp32 values.
+template <int W, int S, bool ANY, bool COUNT, bool REFINE>
 __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
     extern __shared__ uint32_t lds[];
     uint2 spill[kSpillCap];
@@ -83,6 +110,10 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
     int64_t rid = -1;
+    // REFINE: a finished ray's result is stored at the wave's next refill (or at the end), where the
+    // lanes that finished since the last refill refine together; refining as each lane finishes would
+    // run the double-precision code on almost every iteration.  Without refinement a lane stores at once.
+    int64_t done = -1;
     bool exhausted = false;
     uint32_t q_next = 0, q_end = 0;  // the wave's current chunk of ray indices (wave-uniform)
     const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;
@@ -90,10 +121,18 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
     Trav t;
     t.have = false;
     t.top = 0;
+    const auto store = [&](uint32_t i) {
+        if constexpr (REFINE) store_refined(a.sc, t, a.hits, a.counts, i);
+        else store_hit(a.sc, t, a.hits, a.counts, i);
+    };
     for (;;) {
         const unsigned long long idle = __ballot(rid < 0);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (!exhausted && nidle >= a.refill) {
+            if (done >= 0) {
+                store((uint32_t)done);
+                done = -1;
+            }
             if (q_next == q_end) {  // one atomic per chunk keeps the shared counter off the critical path
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 uint32_t base = 0;
@@ -124,10 +163,12 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a) {
             if (on_node) node_step<W, S, ANY, COUNT>(t, m, a.sc.wide);
         }
         if (rid >= 0 && trav_done(t)) {
-            store_hit(a.sc, t, a.hits, a.counts, (uint32_t)rid);
+            if constexpr (REFINE) done = rid;
+            else store((uint32_t)rid);
             rid = -1;
         }
     }
+    if (done >= 0) store((uint32_t)done);
 }
 
 // ---------------------------------------------------------------------------
@@ -196,7 +237,8 @@ __device__ __forceinline__ uint32_t find_item(const BatchItemDev *items, uint32_
     return lo;
 }
 
-template <int W, int S, bool ANY>
+// REFINE: store each hit with t, u, v refined in double (store_refined).
+template <int W, int S, bool ANY, bool REFINE>
 __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
     extern __shared__ uint32_t lds[];
     uint2 spill[kSpillCap];
@@ -204,6 +246,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lt = lane ? (~0ull >> (64u - lane)) : 0ull;
     bool active = false, exhausted = false;
+    bool pending = false;  // REFINE: a finished ray (item, local) whose result waits for the wave's next refill
     uint32_t item = 0, local = 0;
     uint64_t q_next = 0, q_end = 0;  // the wave's current chunk of global ray indices
     uint32_t c_lo = 0, c_hi = 1;     // the items the chunk's rays belong to: [c_lo, c_hi)
@@ -215,10 +258,19 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
     Trav t;
     t.have = false;
     t.top = 0;
+    const auto store = [&]() {
+        const BatchItemDev &it = a.items[item];
+        if constexpr (REFINE) store_refined(it.sc, t, it.hits, nullptr, local);
+        else store_hit(it.sc, t, it.hits, nullptr, local);
+    };
     for (;;) {
         const unsigned long long idle = __ballot(!active);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (!exhausted && nidle >= a.refill) {
+            if (pending) {  // as in k_trace_persistent: the lanes that finished refine together
+                store();
+                pending = false;
+            }
             if (q_next == q_end) {
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 unsigned long long base = 0;
@@ -255,26 +307,12 @@ __global__ __launch_bounds__(256) void k_trace_batch(BatchArgs a) {
             if (on_node) node_step<W, S, ANY, false>(t, m, wide);
         }
         if (active && trav_done(t)) {
-            store_hit(a.items[item].sc, t, a.items[item].hits, nullptr, local);
+            if constexpr (REFINE) pending = true;
+            else store();
             active = false;
         }
     }
-}
-
-// refined results of a fused launch: one thread per global ray index.  One binary search per block
-// finds the item of the block's first ray; a thread then steps over the items that start within the
-// block (few: an item holds whole batches of rays).
-__global__ __launch_bounds__(256) void k_refine_batch(const BatchItemDev *items, uint32_t nitems, uint64_t total) {
-    __shared__ uint32_t first_item;
-    const uint64_t base = (uint64_t)blockIdx.x * blockDim.x;
-    if (threadIdx.x == 0) first_item = find_item(items, 0u, nitems, base);
-    __syncthreads();
-    const uint64_t g = base + threadIdx.x;
-    if (g >= total) return;
-    uint32_t k = first_item;
-    while (k + 1u < nitems && items[k + 1u].first <= g) ++k;
-    const BatchItemDev &it = items[k];
-    refine_one(it.sc.tris, it.sc.sph, it.rays, it.hits, (uint32_t)(g - it.first));
+    if (pending) store();
 }
 
 __global__ void k_zero_counter(unsigned int *c) {
@@ -290,13 +328,16 @@ template <int W, int S> struct Kernels {
     template <bool ANY> static void *direct(bool count) {
         return count ? (void *)k_trace_direct<W, S, ANY, true> : (void *)k_trace_direct<W, S, ANY, false>;
     }
-    template <bool ANY> static void *persistent(bool count) {
-        return count ? (void *)k_trace_persistent<W, S, ANY, true> : (void *)k_trace_persistent<W, S, ANY, false>;
+    template <bool ANY, bool REFINE> static void *persistent(bool count) {
+        return count ? (void *)k_trace_persistent<W, S, ANY, true, REFINE>
+                     : (void *)k_trace_persistent<W, S, ANY, false, REFINE>;
     }
     static void *get(int which, bool any, bool count) {
         if (which == 0) return any ? direct<true>(count) : direct<false>(count);
-        if (which == 1) return any ? persistent<true>(count) : persistent<false>(count);
-        return any ? (void *)k_trace_batch<W, S, true> : (void *)k_trace_batch<W, S, false>;
+        if (which == 1) return any ? persistent<true, false>(count) : persistent<false, false>(count);
+        if (which == 3) return persistent<false, true>(count);  // closest hit only
+        if (which == 2) return any ? (void *)k_trace_batch<W, S, true, false> : (void *)k_trace_batch<W, S, false, false>;
+        return any ? (void *)k_trace_batch<W, S, true, true> : (void *)k_trace_batch<W, S, false, true>;
     }
 };
 
@@ -308,7 +349,8 @@ template <int W> void *pick(uint32_t s, int which, bool any, bool count) {
     }
 }
 
-// which: 0 one lane per ray, 1 persistent, 2 persistent over many scenes (never counts)
+// which: 0 one lane per ray, 1 persistent, 2 persistent over many scenes (never counts), 3 persistent
+// storing refined hits, 4 persistent over many scenes storing refined hits
 void *pick_kernel(uint32_t width, uint32_t s, int which, bool any, bool count) {
     switch (width) {
         case 2: return pick<2>(s, which, any, count);
@@ -353,7 +395,12 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
     if (n == 0) return;
     RT_REQUIRE(rays && hits, "rays/hits must be device pointers");
     check_stack(sc, c);
-    const bool any = (flags & RT_TRACE_ANY) != 0;
+    const bool any = (flags & RT_TRACE_ANY) != 0, refine = (flags & RT_TRACE_FP32) == 0;
+    // Closest-hit persistent launches refine in the kernel, at each wave's refills: measured on one box,
+    // 5.11-5.14 ms per 16M-ray query against 5.19 ms with the separate pass.  Any-hit rays end sooner
+    // and feel the refining kernel's registers (90 VGPRs, 5 waves per SIMD instead of 6) more: 3,270
+    // against 3,410 Mrays/s, so they keep the separate pass.
+    const bool refine_in_kernel = refine && !any && c.persistent;
     TraceArgs a{view_of(sc), rays, hits, counts, n, c.refill, nullptr, c.leaf_weight, nullptr};
     const size_t lds = stack_lds_bytes(c.lds_stack, c.block);
     if (!c.persistent) {
@@ -364,12 +411,12 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
     } else {
         a.work = ctx->counter_slot();
         k_zero_counter<<<1, 64, 0, stream>>>(a.work);
-        void *k = pick_kernel(sc.width, c.lds_stack, 1, any, counts != nullptr);
+        void *k = pick_kernel(sc.width, c.lds_stack, refine_in_kernel ? 3 : 1, any, counts != nullptr);
         uint32_t grid = persistent_grid(ctx, k, c.block, lds);
         void *args[] = {&a};
         RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
     }
-    if (!(flags & RT_TRACE_FP32))
+    if (refine && !refine_in_kernel)
         k_refine_hits<<<(uint32_t)(((uint64_t)n + 255u) / 256u), 256, 0, stream>>>(rays, hits, n, a.sc.tris, a.sc.sph);
     RT_HIP(hipGetLastError());
 }
@@ -423,15 +470,12 @@ void launch_trace_items(rt_context *ctx, const std::vector<TraceItem> &items, ui
                     c.leaf_weight};
         const bool any = (flags & RT_TRACE_ANY) != 0;
         const size_t lds = stack_lds_bytes(c.lds_stack, c.block);
-        void *k = pick_kernel(width, c.lds_stack, 2, any, false);
+        void *k = pick_kernel(width, c.lds_stack, (flags & RT_TRACE_FP32) ? 2 : 4, any, false);
         const uint64_t chunks = (total + kChunk - 1) / kChunk, wpb = c.block / 64u;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(
             1, std::min<uint64_t>(persistent_grid(ctx, k, c.block, lds), (chunks + wpb - 1) / wpb));
         void *args[] = {&a};
         RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
-        if (!(flags & RT_TRACE_FP32))
-            k_refine_batch<<<(uint32_t)((total + 255u) / 256u), 256, 0, stream>>>(
-                static_cast<const BatchItemDev *>(table), (uint32_t)hv.size(), total);
         RT_HIP(hipGetLastError());
         ctx->tables.commit(slot, stream);
     }

@@ -249,7 +249,10 @@ def test_ties_resolve_to_smallest_slot(ctx, native, O):
     ctx.release(sid)
 
 
-def test_batched_launch_equals_single(ctx, native, O):
+@pytest.mark.parametrize("any_hit,fp32", [(False, False), (True, False), (False, True)])
+def test_batched_launch_equals_single(ctx, native, O, any_hit, fp32):
+    """The fused launch (results refined in the kernel, unless fp32) equals each scene's plain launch
+    (closest hit refined in its kernel, any hit by the separate pass) bit for bit."""
     items, ref = [], []
     keep = []
     for k in range(5):
@@ -260,9 +263,9 @@ def test_batched_launch_equals_single(ctx, native, O):
         dr = native.DeviceArray.from_numpy(rays)
         dh = native.DeviceArray(16 * len(rays))
         items.append((sid, len(rays), dr, dh))
-        ref.append(gpu_trace(ctx, sid, rays))
+        ref.append(gpu_trace(ctx, sid, rays, any_hit=any_hit, fp32=fp32))
         keep.append((dr, dh))
-    ctx.trace_batch(items)
+    ctx.trace_batch(items, any_hit=any_hit, fp32=fp32)
     for (sid, n, dr, dh), r in zip(items, ref):
         assert compare_hits(dh.to_numpy(L.HIT, n), r) == 0
         ctx.release(sid)
