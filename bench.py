@@ -433,9 +433,9 @@ def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int
     def pct(a, q):
         return round(float(np.percentile(a, q)), 2) if len(a) else None
 
-    def run(serial: bool, arrivals, reqs=None, sel=None):
+    def run(serial: bool, arrivals, reqs=None, sel=None, no_yield=False):
         srv = N.Server(ctx, arena_bytes=arena, max_active=64, step_us=step_us, serial=serial,
-                       max_spp_step=max_spp_step)
+                       max_spp_step=max_spp_step, no_yield=no_yield)
         warm = requests()[:4]
         srv.replay(warm, np.zeros(len(warm)))
         before = srv.stats()
@@ -455,7 +455,8 @@ def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int
                 "msamples_per_s": round(samples.sum() / (makespan / 1e3) / 1e6, 1),
                 "msegments_per_s": round(segs / (makespan / 1e3) / 1e6, 1),
                 "device_busy_frac": round((after.busy_ms - before.busy_ms) / makespan, 3),
-                "steps": int(after.steps - before.steps), "arena_peak_bytes": int(after.arena_peak)}
+                "steps": int(after.steps - before.steps), "yields": int(after.yields - before.yields),
+                "arena_peak_bytes": int(after.arena_peak)}
 
     # latency of each request served alone: the same server, each request arriving after the previous
     # one has finished (its pixel-samples at 2.5 G/s, well below the measured rate, plus 8 ms)
@@ -477,11 +478,14 @@ def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int
     loads = {}
     for frac in (0.5, 0.8):
         arr = W.poisson_arrivals(nreq, frac * cap, seed=8)
-        loads[str(frac)] = {"qps": round(frac * cap, 3), "batched": run(False, arr), "unbatched": run(True, arr)}
-        b, u = loads[str(frac)]["batched"], loads[str(frac)]["unbatched"]
+        loads[str(frac)] = {"qps": round(frac * cap, 3), "batched": run(False, arr), "unbatched": run(True, arr),
+                            "batched_no_yield": run(False, arr, no_yield=True)}
+        b, u, y = loads[str(frac)]["batched"], loads[str(frac)]["unbatched"], loads[str(frac)]["batched_no_yield"]
         log(f"serving at {frac}x capacity ({frac * cap:.2f} req/s): batched p50 {b['p50_ms']} p99 {b['p99_ms']} ms "
-            f"(small p99 {b['small']['p99_ms']}, device-filling p99 {b['device_filling']['p99_ms']}); unbatched p50 "
-            f"{u['p50_ms']} p99 {u['p99_ms']} ms (small p99 {u['small']['p99_ms']})")
+            f"(small p99 {b['small']['p99_ms']}, device-filling p99 {b['device_filling']['p99_ms']}, "
+            f"{b['yields']} yields); without yielding small p99 {y['small']['p99_ms']}, device-filling p99 "
+            f"{y['device_filling']['p99_ms']}; unbatched p50 {u['p50_ms']} p99 {u['p99_ms']} ms "
+            f"(small p99 {u['small']['p99_ms']})")
     for s in sids:
         ctx.release(s)
     return {"workload": f"{len(sids)} resident scenes ({int(sizes.sum())} triangles, log-uniform 1e4..4194304), "

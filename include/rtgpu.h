@@ -321,6 +321,8 @@ typedef struct rt_server rt_server;
 #define RT_REQ_OUT_ACCUM 0x100u /* render_flags: return the raw accumulation instead of the resolved image */
 
 #define RT_SERVER_SERIAL 1u /* one request per step, whole, through the single-scene kernels (the unbatched baseline) */
+#define RT_SERVER_NO_YIELD 2u /* device-filling renders run their slices to the end while small requests wait
+                                 (by default a slice's last-bounce launch gives the device up to them) */
 
 typedef struct rt_server_config {
     uint64_t arena_bytes;        /* device memory for request buffers: the admission budget (0 = 4 GiB) */
@@ -368,6 +370,7 @@ typedef struct rt_server_stats {
     double busy_ms;        /* device time of all steps */
     double wall_ms;        /* since the server was created */
     uint32_t active, queued;
+    uint64_t yields;       /* device-filling render steps that gave the device up to small requests */
 } rt_server_stats;
 
 int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **out);

@@ -70,6 +70,7 @@ struct RenderArgs {
     uint32_t njobs, nitems, refill, leaf_weight, shade_min;
     unsigned long long *segments;
     unsigned int *work;
+    const uint32_t *gate;  // k_sum_samples: when set, sum only if a finish launch took all its queue (pass_complete)
 };
 
 enum : uint32_t { kIdle = 0, kTracing = 1, kResolved = 2 };
@@ -443,7 +444,17 @@ __device__ __forceinline__ void sum_job(const RenderJob &J) {
     }
 }
 
+// A bounce-by-bounce pass's counters: its two queues' lengths, the finish kernel's work counter
+// and, when that launch yielded, the counter's value at the yield (see k_wave_finish).  The pass
+// is complete once every queued path was taken before any yield.
+constexpr uint32_t kYieldBit = 0x80000000u;  // set in the work counter by a launch that yields
+
+__device__ __forceinline__ bool pass_complete(const uint32_t *count) {
+    return ((count[2] & kYieldBit) ? count[3] : count[2]) >= count[0] + count[1];
+}
+
 template <bool MULTI> __global__ void k_sum_samples(RenderArgs a) {
+    if (a.gate && !pass_complete(a.gate)) return;
     if constexpr (MULTI) sum_job(a.jobs[blockIdx.y]);
     else sum_job(a.job0);
 }
@@ -657,10 +668,25 @@ __global__ __launch_bounds__(kShadeBlock) void k_wave_shade(WaveArgs a) {
 // long to trace; bounce-by-bounce launches would each wait for their longest ray, while
 // here a lane continues its own path as soon as its segment is shaded.  Lanes take queue
 // entries like the ray-query kernel takes rays, and shade in phases like k_render.
+//
+// Yielding: with `yield` set (a counter of yield requests in host memory), the wave that takes
+// every kYieldPoll-th chunk of entries reads it; once it reads `yield_at` or more, it sets
+// kYieldBit in the work counter and records the counter's value before that, work[1].  Every
+// chunk below that value was taken before the bit and is run to its end; a wave whose take
+// returns the bit drops that chunk and takes no more, so the launch ends with the drain of the
+// paths in flight.  The entries left are exactly [work[1], n): k_resume_counter puts the counter
+// back there, and a launch with the same arguments runs them.  Every path still performs the same
+// operations in the same order, so the image does not change.  A server uses this to hand the
+// device to short requests without waiting for the whole launch.  Reads of host memory cross the
+// host link, so only one take in kYieldPoll reads it; the others cost nothing extra.
+constexpr uint32_t kYieldPoll = 32;
+
 struct FinishArgs {
     WaveArgs w;
-    unsigned int *work;
+    unsigned int *work;  // work[0]: the counter; work[1]: its value when the launch yielded
     uint32_t refill, leaf_weight, shade_min;
+    uint32_t *yield;  // host-written yield requests (fine-grained host memory), or nullptr
+    uint32_t yield_at;
 };
 
 template <int W, int S> __global__ __launch_bounds__(256) void k_wave_finish(FinishArgs f) {
@@ -690,9 +716,16 @@ template <int W, int S> __global__ __launch_bounds__(256) void k_wave_finish(Fin
             if (q_next == q_end) {
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 uint32_t base = 0;
-                if ((int)lane == leader) base = atomicAdd(f.work, chunk);
+                if ((int)lane == leader) {
+                    base = atomicAdd(f.work, chunk);
+                    if (f.yield && !(base & kYieldBit) && (base / chunk) % kYieldPoll == 0u &&
+                        __hip_atomic_load(f.yield, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) >= f.yield_at) {
+                        const uint32_t before = atomicOr(f.work, kYieldBit);
+                        if (!(before & kYieldBit)) f.work[1] = before;  // the first to yield records where
+                    }
+                }
                 base = (uint32_t)__shfl((int)base, leader);
-                if (base >= n) exhausted = true;
+                if (base >= n) exhausted = true;  // taken after a yield (kYieldBit), or past the end
                 else { q_next = base; q_end = min(base + chunk, n); }
             }
             if (!exhausted) {
@@ -772,7 +805,8 @@ __global__ void k_accum_clear(float4 *accum, uint32_t npix) {
 }
 
 // SingleTriangleFullScreen.frag:13-21
-__global__ void k_resolve(const float4 *accum, uint32_t npix, uint32_t spp, float4 *out) {
+__global__ void k_resolve(const float4 *accum, uint32_t npix, uint32_t spp, float4 *out, const uint32_t *gate) {
+    if (gate && !pass_complete(gate)) return;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
         float4 a = accum[i];
         float s = (float)spp;
@@ -785,6 +819,11 @@ __global__ void k_resolve(const float4 *accum, uint32_t npix, uint32_t spp, floa
 
 __global__ void k_zero(unsigned int *c) {
     if (threadIdx.x == 0) c[0] = 0u;
+}
+
+// Before a yielded finish launch is resumed: its work counter back to where the yield left it.
+__global__ void k_resume_counter(unsigned int *work) {
+    if (threadIdx.x == 0 && (work[0] & kYieldBit)) work[0] = work[1];
 }
 
 template <int W, bool MULTI> void *pick_render_s(uint32_t s) {
@@ -869,12 +908,13 @@ void launch_accum_clear(float *accum, uint32_t npix, hipStream_t stream) {
     RT_HIP(hipGetLastError());
 }
 
-void launch_resolve(const float *accum, uint32_t npix, uint32_t spp, float *out, hipStream_t stream) {
+void launch_resolve(const float *accum, uint32_t npix, uint32_t spp, float *out, hipStream_t stream,
+                    const uint32_t *gate) {
     if (npix == 0) return;
     RT_REQUIRE(accum && out, "accum/out is NULL");
     uint32_t grid = std::min<uint32_t>((npix + 255) / 256, 65536);
     k_resolve<<<grid, 256, 0, stream>>>(reinterpret_cast<const float4 *>(accum), npix, spp,
-                                        reinterpret_cast<float4 *>(out));
+                                        reinterpret_cast<float4 *>(out), gate);
     RT_HIP(hipGetLastError());
 }
 
@@ -935,15 +975,55 @@ uint32_t wave_finish_bounce() {
 // continuing the carry chain of the previous one.  A pass holds at most wave_paths_per_pass()
 // paths, and no more than `scratch` holds when the caller provides it (else the storage comes
 // from the context's scratch cache).
+// A yieldable pass: its finish launch and gated sum, launched again until the pass is complete.
+struct WaveResume {
+    FinishArgs fa;
+    void *kernel;
+    uint32_t grid, block;
+    size_t lds;
+    RenderArgs sum;
+    uint32_t npix;
+    uint32_t *count;  // the pass's queue lengths and work counter
+};
+
+namespace {
+void launch_finish(const WaveResume &r, hipStream_t stream) {
+    FinishArgs fa = r.fa;
+    void *args[] = {&fa};
+    RT_HIP(hipLaunchKernel(r.kernel, dim3(r.grid), dim3(r.block), args, r.lds, stream));
+    RT_HIP(hipGetLastError());
+}
+}  // namespace
+
+void resume_render_wave(const WaveResume &r, const WaveYield &yield, hipStream_t stream) {
+    WaveResume again = r;
+    again.fa.yield = yield.flag;
+    again.fa.yield_at = yield.yield_at;
+    again.fa.w.segments = nullptr;  // the yielded launch counted these paths' segments as it queued them
+    k_resume_counter<<<1, 64, 0, stream>>>(again.fa.work);
+    RT_HIP(hipGetLastError());
+    launch_finish(again, stream);
+    RenderArgs sa = r.sum;
+    launch_sum_samples(sa, false, r.npix, stream);
+}
+
+const uint32_t *wave_counters(const WaveResume &r) { return r.count; }
+
+bool wave_pass_complete(const uint32_t *c) { return ((c[2] & kYieldBit) ? c[3] : c[2]) >= c[0] + c[1]; }
+
 void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, unsigned long long *segments,
                         const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes,
-                        rt_hit *primary, bool primary_ready) {
+                        rt_hit *primary, bool primary_ready, const WaveYield *yield,
+                        std::shared_ptr<WaveResume> *resume) {
     const uint32_t npix = j.width * j.height;
     uint64_t per_pass = std::min<uint64_t>(j.spp, per_pass_limit(npix));
     if (scratch) {
         RT_REQUIRE(scratch_bytes >= wave_scratch_bytes(npix), "render scratch smaller than one sample of the image");
         per_pass = std::min<uint64_t>(per_pass, std::max<uint64_t>(1, (scratch_bytes - 256) / (wave_scratch_bytes(npix) - 256)));
     }
+    if (resume) resume->reset();
+    // only one pass whose last bounces run in the finish kernel, with the dielectric queue, can yield
+    if (yield && !(resume && per_pass >= j.spp && j.depth > 1 && wave_finish_bounce() == 1)) yield = nullptr;
     const uint64_t maxp = (uint64_t)npix * per_pass;
     RT_REQUIRE(maxp < (1ull << 32), "too many paths in one pass");
     LaunchCfg tc = c;
@@ -958,7 +1038,7 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
     rt_hit *hits = reinterpret_cast<rt_hit *>(rays[1] + maxp);
     uint32_t *ids[2] = {reinterpret_cast<uint32_t *>(hits + maxp), nullptr};
     ids[1] = ids[0] + maxp;
-    uint32_t *count = ids[1] + maxp;  // two queue counters and a work counter
+    uint32_t *count = ids[1] + maxp;  // two queue counters, a work counter and its value at a yield
     const bool cache = !(j.flags & RT_RENDER_NO_PRIMARY_CACHE);
     if (!cache) primary = nullptr;
     const uint32_t block = kShadeBlock;
@@ -1013,16 +1093,22 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
                 a.rays_in = rays[cur];
                 a.ids_in = ids[cur];
                 a.n_in = count + cur;
-                FinishArgs fa{a, count + 2, tc.refill, tc.leaf_weight, tc.shade_min};
-                k_zero<<<1, 64, 0, stream>>>(fa.work);
-                const size_t lds = stack_lds_bytes(tc.lds_stack, tc.block);
-                void *k = pick_finish(sc.width, tc.lds_stack);
+                WaveResume w{};
+                w.fa = FinishArgs{a, count + 2, tc.refill, tc.leaf_weight, tc.shade_min,
+                                  yield ? yield->flag : nullptr, yield ? yield->yield_at : 0u};
+                k_zero<<<1, 64, 0, stream>>>(w.fa.work);
+                w.lds = stack_lds_bytes(tc.lds_stack, tc.block);
+                w.kernel = pick_finish(sc.width, tc.lds_stack);
                 int per_cu = 0;
-                RT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)tc.block, lds));
-                void *args[] = {&fa};
-                RT_HIP(hipLaunchKernel(k, dim3((uint32_t)std::max(1, per_cu) * (uint32_t)ctx->num_cus), dim3(tc.block),
-                                       args, lds, stream));
-                RT_HIP(hipGetLastError());
+                RT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, w.kernel, (int)tc.block, w.lds));
+                w.grid = (uint32_t)std::max(1, per_cu) * (uint32_t)ctx->num_cus;
+                w.block = tc.block;
+                launch_finish(w, stream);
+                if (yield) {
+                    w.npix = npix;
+                    w.count = count;
+                    *resume = std::make_shared<WaveResume>(w);
+                }
                 break;
             }
             launch_trace_queue(ctx, sc, rays[cur], (uint32_t)paths, count + cur, hits, tc, stream);
@@ -1044,6 +1130,10 @@ void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, un
         sa.job0.group = 1;
         sa.job0.samples = col;
         sa.job0.spp = nsamp;
+        if (yield) {  // a yielded pass is summed by the launch that completes it
+            sa.gate = count;
+            (*resume)->sum = sa;
+        }
         launch_sum_samples(sa, false, npix, stream);
     }
 }

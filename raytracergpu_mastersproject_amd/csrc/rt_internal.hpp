@@ -453,9 +453,31 @@ uint64_t per_pass_limit(uint64_t npix);  // samples of an npix-pixel image per p
 // With `primary` (npix hits, caller-owned), the pixels' primary hits are kept there: traced by this call
 // unless `primary_ready`, then read by every pass.  A render cut into several calls traces them once
 // (camera rays carry no jitter, so every sample of a pixel starts with the same hit).
+//
+// Yielding (`yield` set, `resume` receives the pass): a render of one pass whose last bounces run
+// in the finish kernel can give the device up early.  Once *yield->flag reaches yield->yield_at,
+// the finish kernel's waves take no more queue entries and exit when their paths end; the pass's
+// sum (and a resolve gated on wave_counters) then does nothing, and resume_render_wave runs the
+// entries left, as often as it yields in turn.  The image is the same however often it yields.
+// When the render is not one such pass, *resume is left empty and nothing yields.
+struct WaveResume;
+struct WaveYield {
+    uint32_t *flag;     // yield requests, in fine-grained host memory the device reads
+    uint32_t yield_at;  // the launch yields once *flag >= yield_at
+};
 void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, unsigned long long *segments,
                         const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes,
-                        rt_hit *primary = nullptr, bool primary_ready = false);
+                        rt_hit *primary = nullptr, bool primary_ready = false, const WaveYield *yield = nullptr,
+                        std::shared_ptr<WaveResume> *resume = nullptr);
+// Runs the entries a yielded finish launch left (its counter continues where that launch stopped),
+// then the pass's gated sum.  The pass's storage must be as the first launch left it.
+void resume_render_wave(const WaveResume &r, const WaveYield &yield, hipStream_t stream);
+// The pass's device counters (kWaveCounterWords words): its two queues' lengths, the finish
+// kernel's work counter and that counter's value at a yield.  wave_pass_complete tells from a copy
+// of them whether the pass is complete (else it yielded and must be resumed).
+constexpr uint32_t kWaveCounterWords = 4;
+const uint32_t *wave_counters(const WaveResume &r);
+bool wave_pass_complete(const uint32_t *counters);
 // With `scratch` set, every buffer beyond the inputs and outputs comes from it (scratch_bytes long):
 // bounce-by-bounce renders run in as many passes as it needs (RT_ERR_BUDGET when one sample of the
 // image does not fit), and persistent launches whose per-sample storage does not fit run whole pixels
@@ -471,7 +493,9 @@ RenderJob make_render_job(const Scene &sc, const rt_params &p, uint32_t width, u
 // One launch over all jobs (all scenes must have BVH width `bvh_width`); returns the work items launched.
 uint64_t launch_render_jobs(rt_context *ctx, std::vector<RenderJob> &jobs, uint32_t bvh_width,
                             unsigned long long *segments, const LaunchCfg &cfg, hipStream_t stream);
-void launch_resolve(const float *accum, uint32_t npix, uint32_t spp, float *out, hipStream_t stream);
+// With `gate` (a yieldable pass's counters, WaveResume::counters), nothing is written unless that pass is complete.
+void launch_resolve(const float *accum, uint32_t npix, uint32_t spp, float *out, hipStream_t stream,
+                    const uint32_t *gate = nullptr);
 
 // host camera (raytraceBVH.comp:49-81); out: pixel00, deltaU, deltaV, origin
 void camera_setup(const rt_params &p, uint32_t width, uint32_t height, float out[12]);

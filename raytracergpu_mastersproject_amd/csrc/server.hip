@@ -12,8 +12,12 @@
 //     (launch_render_jobs, split layout), whole where they fit the budget, plus one fused
 //     ray-query launch over slices of every admitted query (launch_trace_items).
 // The lanes' kernels share the device: a small step's kernels take compute units as the big
-// lane's bounce kernels release them, so a small request does not wait for a device-filling
-// step to end.  A request keeps its lane for life, so its steps stay in order on one stream.
+// lane's bounce kernels release them.  A big render's last bounces run in one persistent
+// launch; while small requests keep arriving its slices are single yieldable passes, and
+// enqueuing a small step asks that launch to give the device up (its waves finish the paths
+// they hold and exit).  The pass is resumed on the big lane after the small step, so a small
+// request waits for the drain of the paths in flight, not for a whole slice.  A request keeps
+// its lane for life, so its steps stay in order on one stream.
 // A step's path state or per-sample storage comes from the second half of the arena for the
 // step's lifetime.
 // A request's last slice is followed by its resolve; when that step's event completes
@@ -108,6 +112,8 @@ struct Request {
 
 struct Step {
     hipEvent_t start = nullptr, end = nullptr;
+    std::shared_ptr<WaveResume> resume;  // a yieldable pass: resumed at retirement if it yielded
+    uint32_t *status = nullptr;          // host copy of that pass's counters, written before `end`
     std::vector<std::shared_ptr<Request>> finishing;
     std::vector<std::shared_ptr<Request>> members;
     uint64_t samples = 0, rays = 0;
@@ -115,6 +121,8 @@ struct Step {
     bool big_lane = false;    // enqueued on the device-filling renders' stream
     bool overlapped = false;  // shared the device with a step of the other stream
 };
+
+constexpr uint32_t kStatusSlots = 64;  // status slots of yieldable passes (at most two are ever in use)
 
 }  // namespace
 }  // namespace rtgpu
@@ -127,6 +135,14 @@ struct rt_server {
     rtgpu::Arena scratch;  // second half: per-sample storage of the steps in flight
     uint64_t scratch_base = 0;
     rtgpu::DevBuf segments;  // u64 device counter of traced render segments
+    // Fine-grained host memory: [0] counts yield requests (the big lane's yieldable finish launches
+    // read it), then kStatusSlots slots of 4 words that receive yieldable passes' counters.
+    uint32_t *pinned = nullptr;
+    uint32_t *pinned_dev = nullptr;  // its device address
+    uint32_t yield_gen = 0;          // the value in pinned[0]
+    uint32_t status_next = 0;
+    bool yield_pending = false;      // a small step was enqueued while a yieldable pass was in flight
+    double yield_min_ms = 2.0;       // a pass runs this long before a small step may make it yield
     // Two lanes: device-filling renders run on `stream`, everything else (small renders, ray
     // queries) on `small_stream`, so a small request never waits behind a device-filling step:
     // its kernels take compute units as the other lane's kernels release them.  The unbatched
@@ -196,6 +212,55 @@ hipEvent_t take_event(rt_server *s) {
 }
 
 char *arena_ptr(rt_server *s, uint64_t off) { return static_cast<char *>(s->arena_mem.p) + off; }
+
+uint32_t *take_status(rt_server *s) {
+    static_assert(kWaveCounterWords <= 4, "a status slot holds 4 words");
+    return s->pinned + 4 + 4 * (s->status_next++ % kStatusSlots);
+}
+
+bool yieldable_inflight(const rt_server *s) {
+    for (const Step &st : s->inflight)
+        if (st.big_lane && st.resume) return true;
+    return false;
+}
+
+// Asks the big lane's yieldable finish launches enqueued so far to give the device up.
+void request_yield(rt_server *s) {
+    s->yield_gen++;
+    __atomic_store_n(s->pinned, s->yield_gen, __ATOMIC_SEQ_CST);
+}
+
+// How long the in-flight yieldable pass has run on the device (0 before it starts; the device
+// clock of its start event against the server's origin event, which is also the host clock's).
+double yieldable_run_ms(rt_server *s) {
+    for (const Step &st : s->inflight) {
+        if (!st.big_lane || !st.resume) continue;
+        float a = 0.0f;
+        if (hipEventQuery(st.start) != hipSuccess || hipEventElapsedTime(&a, s->t0_event, st.start) != hipSuccess)
+            return 0.0;
+        return std::max(0.0, s->now_ms() - (double)a);
+    }
+    return 0.0;
+}
+
+// The big lane's next work starts after the small steps in flight (the small lane runs in order,
+// so its last step's end covers them).  Called with the context lock held.
+void after_small_steps(rt_server *s) {
+    for (auto it = s->inflight.rbegin(); it != s->inflight.rend(); ++it)
+        if (!it->big_lane) {
+            RT_HIP(hipStreamWaitEvent(s->stream, it->end, 0));
+            return;
+        }
+}
+
+void resolve_finishing(rt_server *s, const std::vector<std::shared_ptr<Request>> &finishing, hipStream_t stream,
+                       const uint32_t *gate) {
+    for (auto &r : finishing) {
+        if (r->req.kind != RT_REQ_RENDER || (r->req.render_flags & RT_REQ_OUT_ACCUM)) continue;
+        float *accum = reinterpret_cast<float *>(arena_ptr(s, r->offset));
+        launch_resolve(accum, r->req.width * r->req.height, r->req.spp, accum, stream, gate);  // in place
+    }
+}
 
 hipStream_t lane_stream(const rt_server *s, bool big_lane) {
     return big_lane || (s->cfg.flags & RT_SERVER_SERIAL) ? s->stream : s->small_stream;
@@ -279,9 +344,10 @@ bool enqueue_step(rt_server *s, bool big_lane) {
     auto npix_of = [](const Request &r) { return (uint64_t)r.req.width * r.req.height; };
     const bool small_recent = s->now_ms() - s->last_small_ms < 2.0 * s->cfg.step_us / 1000.0;
     const bool big_step = !serial && big_lane && !renders.empty();
+    const bool yields = !serial && !(s->cfg.flags & RT_SERVER_NO_YIELD);
     if (big_step) renders.resize(1);
     else if (!serial && !renders.empty()) budget = std::max(1.0, budget / 5.0);
-    if (big_step && small_recent) budget = std::max(1.0, budget / s->slice_div);
+    if (big_step && small_recent && !yields) budget = std::max(1.0, budget / s->slice_div);
     for (auto &r : renders) {
         const uint64_t npix = npix_of(*r);
         uint32_t q = r->remaining;
@@ -306,6 +372,8 @@ bool enqueue_step(rt_server *s, bool big_lane) {
     if (wave) {  // halve the slice until its path storage fits; else the persistent launch below
         auto &[r, q] = plan[0];
         const uint64_t npix = npix_of(*r);
+        // while small requests arrive, one pass per step: its finish launch can yield to them
+        if (yields && small_recent) q = (uint32_t)std::min<uint64_t>(q, per_pass_limit(npix));
         auto need = [&](uint64_t samples) { return wave_scratch_bytes(npix * std::min(samples, per_pass_limit(npix))); };
         uint32_t qq = q;
         bool fits = s->scratch.alloc(need(qq), scratch_off);
@@ -342,6 +410,7 @@ bool enqueue_step(rt_server *s, bool big_lane) {
         step.scratch_bytes = wave ? scratch_bytes : 16ull * total;
     }
     std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+    const bool yieldable = wave && yields && small_recent;
     RT_HIP(hipEventRecord(step.start, stream));
     const uint32_t group = (uint32_t)std::max<uint64_t>(1, total / (32ull * device_lanes(s)));
     char *storage = static_cast<char *>(s->arena_mem.p) + s->scratch_base + scratch_off;
@@ -365,8 +434,10 @@ bool enqueue_step(rt_server *s, bool big_lane) {
             RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
             // the primary hits, kept after the image, are traced by the render's first bounce-by-bounce slice
             rt_hit *primary = reinterpret_cast<rt_hit *>(arena_ptr(s, r->offset + Arena::round(16ull * npix)));
+            const WaveYield wy{s->pinned_dev, s->yield_gen + 1};  // yields to small steps enqueued from now on
             launch_render_wave(s->ctx, *r->scene, j, s->segments.as<unsigned long long>(), rc, stream, storage,
-                               scratch_bytes, r->big_lane ? primary : nullptr, r->primary_ready);
+                               scratch_bytes, r->big_lane ? primary : nullptr, r->primary_ready,
+                               yieldable ? &wy : nullptr, yieldable ? &step.resume : nullptr);
             r->primary_ready = r->big_lane;
         } else {
             RenderJob j = make_render_job(*r->scene, r->req.params, r->req.width, r->req.height, q, kflags, accum);
@@ -412,11 +483,14 @@ bool enqueue_step(rt_server *s, bool big_lane) {
     for (auto &r : step.members) {
         if (r->remaining) continue;
         r->state = State::Finishing;
-        if (r->req.kind == RT_REQ_RENDER && !(r->req.render_flags & RT_REQ_OUT_ACCUM)) {
-            float *accum = reinterpret_cast<float *>(arena_ptr(s, r->offset));
-            launch_resolve(accum, r->req.width * r->req.height, r->req.spp, accum, stream);  // in place
-        }
         step.finishing.push_back(r);
+    }
+    // a yieldable pass's resolve runs only if the pass completed; otherwise its resume step's does
+    resolve_finishing(s, step.finishing, stream, step.resume ? wave_counters(*step.resume) : nullptr);
+    if (step.resume) {
+        step.status = take_status(s);
+        RT_HIP(hipMemcpyAsync(step.status, wave_counters(*step.resume), kWaveCounterWords * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, stream));
     }
     RT_HIP(hipEventRecord(step.end, stream));
     s->stats.steps++;
@@ -443,11 +517,44 @@ double uncovered_ms(const std::deque<std::pair<float, float>> &recent, float a, 
     return std::max(0.0, (double)(b - a) - covered);
 }
 
+// A yielded pass goes on in a step of its own on the big lane, after the small steps in flight; the
+// requests it finishes and its path storage move to that step.  Called with s->m held.
+void resume_step(rt_server *s, Step &from) {
+    Step rs;
+    rs.start = take_event(s);
+    rs.end = take_event(s);
+    rs.resume = from.resume;
+    rs.finishing = std::move(from.finishing);
+    from.finishing.clear();
+    rs.scratch_off = from.scratch_off;
+    rs.scratch_bytes = from.scratch_bytes;
+    from.scratch_bytes = 0;
+    rs.big_lane = true;
+    rs.status = take_status(s);
+    {
+        std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+        after_small_steps(s);
+        RT_HIP(hipEventRecord(rs.start, s->stream));
+        resume_render_wave(*rs.resume, WaveYield{s->pinned_dev, s->yield_gen + 1}, s->stream);
+        resolve_finishing(s, rs.finishing, s->stream, wave_counters(*rs.resume));
+        RT_HIP(hipMemcpyAsync(rs.status, wave_counters(*rs.resume), kWaveCounterWords * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s->stream));
+        RT_HIP(hipEventRecord(rs.end, s->stream));
+    }
+    s->stats.steps++;
+    s->stats.yields++;
+    s->last_big_ms = s->now_ms();
+    for (Step &o : s->inflight)
+        if (!o.big_lane) o.overlapped = rs.overlapped = true;
+    s->inflight.push_back(std::move(rs));
+}
+
 // Completes inflight[i] (its end event has fired).  Called with s->m held.  A lane's steps retire
 // in order, so a request's last step retires after its others.
 void retire_step(rt_server *s, size_t i, double done_ms) {
     Step step = std::move(s->inflight[i]);
     s->inflight.erase(s->inflight.begin() + (std::ptrdiff_t)i);
+    if (step.resume && !wave_pass_complete(step.status)) resume_step(s, step);
     float ms = 0.0f, a = 0.0f, b = 0.0f;
     if (hipEventElapsedTime(&ms, step.start, step.end) == hipSuccess && ms > 0.0f &&
         hipEventElapsedTime(&a, s->t0_event, step.start) == hipSuccess &&
@@ -526,14 +633,31 @@ void worker_main(rt_server *s) {
             if (serial) {
                 progress = s->inflight.size() < s->cfg.pipeline && enqueue_step(s, true);
             } else {
-                if (lane_inflight(s, false) < s->cfg.pipeline && enqueue_step(s, false)) progress = true;
-                // While small requests keep arriving, the big lane runs one short slice at a time and
-                // waits while a small step runs, unless it has waited a whole step budget already.
+                if (lane_inflight(s, false) < s->cfg.pipeline && enqueue_step(s, false)) {
+                    progress = true;
+                    if (yieldable_inflight(s)) s->yield_pending = true;
+                }
+                // A yieldable pass gives the device up to the small steps enqueued since it started
+                // once it has run yield_min_ms (so a stream of small steps cannot starve it); it is
+                // resumed after them (resume_step).
+                if (s->yield_pending) {
+                    if (!yieldable_inflight(s)) {
+                        s->yield_pending = false;
+                    } else if (yieldable_run_ms(s) >= s->yield_min_ms) {
+                        request_yield(s);
+                        s->yield_pending = false;
+                    }
+                }
+                // While small requests keep arriving, the big lane runs one slice at a time, and none is
+                // enqueued while a small step runs unless the big lane has waited a whole step budget
+                // already.  Its slices are single yieldable passes; with RT_SERVER_NO_YIELD they are
+                // short instead.
                 const double now = s->now_ms();
                 const bool small_recent = now - s->last_small_ms < 2.0 * s->cfg.step_us / 1000.0;
                 const bool held = small_recent && lane_inflight(s, false) > 0 &&
                                   now - s->last_big_ms < s->cfg.step_us / 1000.0;
-                if (!held && lane_inflight(s, true) < (small_recent ? 1u : s->cfg.pipeline) && enqueue_step(s, true)) {
+                const bool one = small_recent || yieldable_inflight(s);  // a yielded pass resumes before the next
+                if (!held && lane_inflight(s, true) < (one ? 1u : s->cfg.pipeline) && enqueue_step(s, true)) {
                     s->last_big_ms = now;
                     progress = true;
                 }
@@ -541,11 +665,12 @@ void worker_main(rt_server *s) {
             if (progress) continue;
             if (!s->inflight.empty()) {
                 // Wait for the oldest small-lane step when one is in flight (it is short).  With only
-                // device-filling steps in flight, poll instead, so a request submitted meanwhile
-                // starts at once on the small lane rather than after the device-filling step.
+                // device-filling steps in flight, or a yield to request, poll instead, so a request
+                // submitted meanwhile starts at once on the small lane rather than after the
+                // device-filling step.
                 size_t w = 0;
                 while (w + 1 < s->inflight.size() && s->inflight[w].big_lane && !serial) ++w;
-                if (serial || !s->inflight[w].big_lane) {
+                if (serial || (!s->inflight[w].big_lane && !s->yield_pending)) {
                     hipEvent_t e = s->inflight[w].end;
                     lk.unlock();
                     const hipError_t err = hipEventSynchronize(e);
@@ -569,6 +694,8 @@ void worker_main(rt_server *s) {
 }
 
 void destroy_streams(rt_server *s) {
+    if (s->pinned) (void)hipHostFree(s->pinned);  // after the streams' work (the caller synchronised them)
+    s->pinned = s->pinned_dev = nullptr;
     if (s->t0_event) (void)hipEventDestroy(s->t0_event);
     if (s->small_stream) (void)hipStreamDestroy(s->small_stream);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -596,7 +723,7 @@ int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **o
         if (!s->cfg.step_us) s->cfg.step_us = 50000;
         if (!s->cfg.pipeline) s->cfg.pipeline = 2;
         s->cfg.pipeline = std::min(s->cfg.pipeline, 8u);
-        RT_REQUIRE((s->cfg.flags & ~RT_SERVER_SERIAL) == 0, "unknown server flags");
+        RT_REQUIRE((s->cfg.flags & ~(RT_SERVER_SERIAL | RT_SERVER_NO_YIELD)) == 0, "unknown server flags");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         RT_HIP(hipSetDevice(ctx->device));
         s->arena_mem.alloc(s->cfg.arena_bytes);
@@ -605,10 +732,15 @@ int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **o
         s->scratch.reset(s->cfg.arena_bytes - s->scratch_base);
         s->segments.alloc(8);
         RT_HIP(hipMemset(s->segments.p, 0, 8));
+        const size_t pinned_bytes = sizeof(uint32_t) * (4 + 4 * kStatusSlots);
+        RT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->pinned), pinned_bytes, hipHostMallocCoherent));
+        std::memset(s->pinned, 0, pinned_bytes);
+        RT_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&s->pinned_dev), s->pinned, 0));
         // non-blocking: the server's work does not serialise with the legacy default stream
         RT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         RT_HIP(hipStreamCreateWithFlags(&s->small_stream, hipStreamNonBlocking));
         if (const char *v = std::getenv("RTGPU_SERVER_SLICE_DIV")) s->slice_div = std::max(1.0, std::atof(v));
+        if (const char *v = std::getenv("RTGPU_SERVER_YIELD_MIN_MS")) s->yield_min_ms = std::max(0.0, std::atof(v));
         RT_HIP(hipEventCreate(&s->t0_event));
         RT_HIP(hipEventRecord(s->t0_event, s->stream));
         s->stats.arena_bytes = s->arena.size();

@@ -83,6 +83,7 @@ REQ_RENDER = 0
 REQ_TRACE = 1
 REQ_OUT_ACCUM = 0x100
 SERVER_SERIAL = 1
+SERVER_NO_YIELD = 2
 
 
 class ServerConfig(C.Structure):
@@ -107,7 +108,7 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64), ("steps", C.c_uint64),
                 ("samples", C.c_uint64), ("segments", C.c_uint64), ("rays", C.c_uint64),
                 ("arena_bytes", C.c_uint64), ("arena_peak", C.c_uint64), ("busy_ms", C.c_double),
-                ("wall_ms", C.c_double), ("active", C.c_uint32), ("queued", C.c_uint32)]
+                ("wall_ms", C.c_double), ("active", C.c_uint32), ("queued", C.c_uint32), ("yields", C.c_uint64)]
 
 
 _vp, _u32, _i32, _sz = C.c_void_p, C.c_uint32, C.c_int, C.c_size_t
@@ -472,8 +473,10 @@ class Server:
     """A serving loop over a context's resident scenes (rt_server_*)."""
 
     def __init__(self, ctx: Context, arena_bytes=1 << 30, max_active=64, step_us=50000, pipeline=2, serial=False,
-                 max_spp_step=0, render_cfg: LaunchConfig | None = None, trace_cfg: LaunchConfig | None = None):
-        cfg = ServerConfig(arena_bytes, max_active, step_us, pipeline, SERVER_SERIAL if serial else 0, max_spp_step)
+                 max_spp_step=0, render_cfg: LaunchConfig | None = None, trace_cfg: LaunchConfig | None = None,
+                 no_yield=False):
+        flags = (SERVER_SERIAL if serial else 0) | (SERVER_NO_YIELD if no_yield else 0)
+        cfg = ServerConfig(arena_bytes, max_active, step_us, pipeline, flags, max_spp_step)
         if render_cfg:
             cfg.render_cfg = render_cfg
         if trace_cfg:

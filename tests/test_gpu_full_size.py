@@ -118,7 +118,7 @@ def test_serving_at_full_size(ctx, native):
     benchmark's 160 render requests (256^2, 512^2 or 1920x1080, 1..64 spp, depth 8, seed = request id),
     plus 16 ray-query requests of 2^20 rays, replayed at Poisson arrivals through the continuous-batching
     server with a 16 GiB arena.  Every served result equals the request run alone (rt_render or
-    rt_trace), bit for bit."""
+    rt_trace), bit for bit, including the device-filling renders whose passes yielded to small requests."""
     sizes = W.serving_scene_sizes()
     sids, params, bounds = [], [], []
     for k, nt in enumerate(sizes):
@@ -146,8 +146,9 @@ def test_serving_at_full_size(ctx, native):
     stats = srv.replay(reqs, W.poisson_arrivals(len(reqs), 160.0))
     st = srv.stats()
     srv.close()
-    print(f"served {st.completed} requests in {st.steps} steps", flush=True)
+    print(f"served {st.completed} requests in {st.steps} steps ({st.yields} yielded passes resumed)", flush=True)
     assert all(s.status == 0 for s in stats) and st.completed == len(reqs)
+    assert st.yields > 0  # device-filling passes gave the device up to small requests and were resumed
     big = 0
     for o in outs:
         if o[0] == "render":

@@ -328,13 +328,14 @@ def test_big_request_runs_bounce_by_bounce(ctx, native, scenes):
         assert np.array_equal(out.reshape(-1).view(np.uint32), expect.view(np.uint32))
 
 
-@pytest.mark.parametrize("slice_div", [None, "1", "32"])
-def test_big_and_small_requests_mixed(ctx, native, scenes, monkeypatch, slice_div):
+@pytest.mark.parametrize("yields,slice_div", [(True, None), (False, None), (False, "1"), (False, "32")])
+def test_big_and_small_requests_mixed(ctx, native, scenes, monkeypatch, yields, slice_div):
     """Two device-filling renders (bounce-by-bounce steps on the big lane) submitted among small
     renders and ray queries (fused steps on the small lane): every result equals the request run
-    alone.  The big lane's slice while small requests arrive is the default eighth of the step
-    budget, the whole budget, or a 32nd (many slices, each reading the render's cached primary
-    hits)."""
+    alone.  By default the big lane's passes may yield to the small steps and are resumed after them.
+    With RT_SERVER_NO_YIELD its slice while small requests arrive is the
+    default eighth of the step budget, the whole budget, or a 32nd (many slices, each reading the
+    render's cached primary hits)."""
     if slice_div is not None:
         monkeypatch.setenv("RTGPU_SERVER_SLICE_DIV", slice_div)
     ids, params, soup, ico = scenes
@@ -345,11 +346,52 @@ def test_big_and_small_requests_mixed(ctx, native, scenes, monkeypatch, slice_di
         k = len(reqs) // 2  # in the middle of the stream of small requests
         reqs.insert(k, native.render_request(ids[name], p, w, h, spp, native.REQ_OUT_ACCUM, out=out))
         expect.append(("render", name, p, w, h, spp, 0, out))
-    srv = native.Server(ctx, arena_bytes=3 << 30, max_active=8, step_us=20000)
+    srv = native.Server(ctx, arena_bytes=3 << 30, max_active=8, step_us=20000, no_yield=not yields)
     stats = srv.replay(reqs, np.arange(len(reqs), dtype=np.float64) * 0.5)
     st = srv.stats()
     srv.close()
     assert all(s.status == 0 for s in stats) and st.completed == len(reqs)
+    if not yields:
+        assert st.yields == 0
     if slice_div == "32":
         assert max(s.steps for s in stats) > 4  # the big renders went in many slices
     _check_results(native, ctx, ids, expect)
+
+
+def test_device_filling_render_yields_to_small_requests(ctx, native, scenes, monkeypatch):
+    """A device-filling render (a 200,000-triangle displaced icosphere at 1920x1080, 32 spp: two
+    bounce-by-bounce passes) with small renders arriving every half millisecond while it runs: its
+    passes give the device up to the small steps (counted in the server's yields) and are resumed
+    after them, as often as small steps arrive (RTGPU_SERVER_YIELD_MIN_MS=0: however briefly a pass
+    has run).  Every image equals the request rendered alone, and the small requests finish before
+    the big one."""
+    monkeypatch.setenv("RTGPU_SERVER_YIELD_MIN_MS", "0")
+    ids, params, _, _ = scenes
+    big = W.icosphere_scene(freq=100)
+    ids = dict(ids, big=ctx.upload(big["tris"], None, big["materials"]))
+    big_p = with_state(big["params"], 77)
+    big_out = np.zeros((1920 * 1080, 4), np.float32)
+    reqs = [native.render_request(ids["big"], big_p, 1920, 1080, 32, native.REQ_OUT_ACCUM, out=big_out)]
+    expect = [("render", "big", big_p, 1920, 1080, 32, 0, big_out)]
+    for k in range(24):
+        name = ("soup4", "ico", "ico2")[k % 3]
+        p = with_state(params[name], 100 + k)
+        out = np.zeros((64 * 48, 4), np.float32)
+        reqs.append(native.render_request(ids[name], p, 64, 48, 4, native.REQ_OUT_ACCUM, out=out))
+        expect.append(("render", name, p, 64, 48, 4, 0, out))
+    # the big render arrives among the small ones, so its passes run while small requests keep coming
+    reqs.insert(1, reqs.pop(0))
+    expect.insert(1, expect.pop(0))
+    arrivals = np.concatenate([[0.0, 0.25], 0.5 * np.arange(1, 24)])
+    srv = native.Server(ctx, arena_bytes=3 << 30, max_active=16, step_us=20000)
+    stats = srv.replay(reqs, arrivals)
+    st = srv.stats()
+    srv.close()
+    lat = [s.done_ms - s.submit_ms for s in stats]
+    small = lat[:1] + lat[2:]
+    print(f"yields {st.yields}, steps {st.steps}; big {lat[1]:.2f} ms, small max {max(small):.2f} ms", flush=True)
+    assert all(s.status == 0 for s in stats) and st.completed == len(reqs)
+    assert st.yields > 0
+    assert max(small) < lat[1]
+    _check_results(native, ctx, ids, expect)
+    ctx.release(ids["big"])
