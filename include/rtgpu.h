@@ -370,7 +370,7 @@ typedef struct rt_server_stats {
     double busy_ms;        /* device time of all steps */
     double wall_ms;        /* since the server was created */
     uint32_t active, queued;
-    uint64_t yields;       /* device-filling render steps that gave the device up to small requests */
+    uint64_t yields;       /* passes of device-filling renders resumed after yielding to small requests */
 } rt_server_stats;
 
 int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **out);

@@ -142,7 +142,7 @@ struct rt_server {
     uint32_t yield_gen = 0;          // the value in pinned[0]
     uint32_t status_next = 0;
     bool yield_pending = false;      // a small step was enqueued while a yieldable pass was in flight
-    double yield_min_ms = 2.0;       // a pass runs this long before a small step may make it yield
+    double yield_min_ms = 0.5;       // a pass runs this long before a small step may make it yield
     // Two lanes: device-filling renders run on `stream`, everything else (small renders, ray
     // queries) on `small_stream`, so a small request never waits behind a device-filling step:
     // its kernels take compute units as the other lane's kernels release them.  The unbatched
