@@ -971,10 +971,6 @@ uint32_t wave_finish_bounce() {
 }
 }  // namespace
 
-// Bounce-by-bounce rendering of one image: passes of whole samples of every pixel, each
-// continuing the carry chain of the previous one.  A pass holds at most wave_paths_per_pass()
-// paths, and no more than `scratch` holds when the caller provides it (else the storage comes
-// from the context's scratch cache).
 // A yieldable pass: its finish launch and gated sum, launched again until the pass is complete.
 struct WaveResume {
     FinishArgs fa;
@@ -983,7 +979,7 @@ struct WaveResume {
     size_t lds;
     RenderArgs sum;
     uint32_t npix;
-    uint32_t *count;  // the pass's queue lengths and work counter
+    uint32_t *count;  // the pass's queue lengths, work counter and the counter's value at a yield
 };
 
 namespace {
@@ -999,7 +995,6 @@ void resume_render_wave(const WaveResume &r, const WaveYield &yield, hipStream_t
     WaveResume again = r;
     again.fa.yield = yield.flag;
     again.fa.yield_at = yield.yield_at;
-    again.fa.w.segments = nullptr;  // the yielded launch counted these paths' segments as it queued them
     k_resume_counter<<<1, 64, 0, stream>>>(again.fa.work);
     RT_HIP(hipGetLastError());
     launch_finish(again, stream);
@@ -1011,6 +1006,10 @@ const uint32_t *wave_counters(const WaveResume &r) { return r.count; }
 
 bool wave_pass_complete(const uint32_t *c) { return ((c[2] & kYieldBit) ? c[3] : c[2]) >= c[0] + c[1]; }
 
+// Bounce-by-bounce rendering of one image: passes of whole samples of every pixel, each
+// continuing the carry chain of the previous one.  A pass holds at most wave_paths_per_pass()
+// paths, and no more than `scratch` holds when the caller provides it (else the storage comes
+// from the context's scratch cache).
 void launch_render_wave(rt_context *ctx, const Scene &sc, const RenderJob &j, unsigned long long *segments,
                         const LaunchCfg &c, hipStream_t stream, char *scratch, uint64_t scratch_bytes,
                         rt_hit *primary, bool primary_ready, const WaveYield *yield,

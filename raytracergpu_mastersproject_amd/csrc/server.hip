@@ -14,10 +14,10 @@
 // The lanes' kernels share the device: a small step's kernels take compute units as the big
 // lane's bounce kernels release them.  A big render's last bounces run in one persistent
 // launch; while small requests keep arriving its slices are single yieldable passes, and
-// enqueuing a small step asks that launch to give the device up (its waves finish the paths
-// they hold and exit).  The pass is resumed on the big lane after the small step, so a small
-// request waits for the drain of the paths in flight, not for a whole slice.  A request keeps
-// its lane for life, so its steps stay in order on one stream.
+// enqueuing a small step asks that launch to give the device up once the pass has run
+// yield_min_ms (its waves finish the paths they hold and exit).  The pass is resumed on the big
+// lane after the small step, so a small request waits for the drain of the paths in flight, not
+// for a whole slice.  A request keeps its lane for life, so its steps stay in order on one stream.
 // A step's path state or per-sample storage comes from the second half of the arena for the
 // step's lifetime.
 // A request's last slice is followed by its resolve; when that step's event completes
