@@ -166,11 +166,11 @@ struct rt_server {
     // rates measured from retired steps (work units per microsecond of device time)
     double sample_rate = 1000.0, ray_rate = 2000.0;
     // host time (ms since creation) of the last admission of a request that is not a device-filling
-    // render: while such requests keep arriving, a device-filling render that runs as one
-    // persistent launch (not bounce by bounce) advances in short slices
+    // render: while such requests keep arriving, a device-filling render advances one yieldable
+    // pass at a time (or, with RT_SERVER_NO_YIELD, in short slices)
     double last_small_ms = -1e30;
     double last_big_ms = -1e30;  // host time of the last big-lane step enqueued
-    double slice_div = 8.0;      // big-lane slices while small requests arrive: the budget / slice_div
+    double slice_div = 8.0;      // RT_SERVER_NO_YIELD: big-lane slices while small requests arrive, budget / slice_div
     rt_server_stats stats{};
     rtgpu::Clock::time_point t0;
     std::thread worker;
@@ -333,7 +333,8 @@ bool enqueue_step(rt_server *s, bool big_lane) {
     // admission, rt_render's own threshold) runs alone, bounce by bounce as rt_render would run
     // it (below), in slices of the budget.  Its last bounces run in one persistent launch that
     // holds the compute units to its end, so while small requests keep arriving (one admitted
-    // within the last two step budgets) its slices are cut to budget / slice_div, and the worker
+    // within the last two step budgets) its slices are single passes whose finish launch yields
+    // to small steps (or, with RT_SERVER_NO_YIELD, slices of budget / slice_div), and the worker
     // keeps at most one of them in flight and none while a small step runs (worker_main).
     //
     // Storage (path state, or per-sample colours) comes from the scratch half of the arena and
