@@ -40,9 +40,20 @@ struct TraceArgs {
     const uint32_t *n_dev;  // when set, the ray count is min(*n_dev, n)
 };
 
+// Rays are read once and hits written once per launch: non-temporal accesses keep them from
+// displacing the BVH nodes and leaf records in L2 and the MALL (refined closest-hit query on
+// 4M triangles: 5.10 against 5.13 ms).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int nt_u4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void store_hit_nt(rt_hit *hits, uint32_t i, uint4 h) {
+    const nt_u4 v = {h.x, h.y, h.z, h.w};
+    __builtin_nontemporal_store(v, reinterpret_cast<nt_u4 *>(hits) + i);
+}
+
 __device__ __forceinline__ void load_ray(const rt_ray *rays, uint32_t i, v3 &o, float &tmin, v3 &d, float &tmax) {
-    const float4 *p = reinterpret_cast<const float4 *>(rays + i);
-    float4 a = p[0], b = p[1];
+    const nt_f4 *p = reinterpret_cast<const nt_f4 *>(rays + i);
+    const nt_f4 a = __builtin_nontemporal_load(p), b = __builtin_nontemporal_load(p + 1);
     o = v3{a.x, a.y, a.z};
     tmin = a.w;
     d = v3{b.x, b.y, b.z};
@@ -55,7 +66,7 @@ __device__ __forceinline__ void store_hit(const SceneView &sc, const Trav &t, rt
     if (t.bslot == kEmpty) h = make_uint4(RT_NO_HIT, __float_as_uint(t.bt), 0u, 0u);
     else h = make_uint4(sc.slot_prim ? sc.slot_prim[t.bslot] : t.bslot, __float_as_uint(t.bt), __float_as_uint(t.bu),
                         __float_as_uint(t.bv));
-    reinterpret_cast<uint4 *>(hits)[i] = h;
+    store_hit_nt(hits, i, h);
     if (counts) reinterpret_cast<uint2 *>(counts)[i] = make_uint2(t.nbox, t.ntri);
 }
 
@@ -97,7 +108,7 @@ __device__ __forceinline__ void store_refined(const SceneView &sc, const Trav &t
         }
         h = make_uint4(prim, __float_as_uint(tt), __float_as_uint(u), __float_as_uint(v));
     }
-    reinterpret_cast<uint4 *>(hits)[i] = h;
+    store_hit_nt(hits, i, h);
     if (counts) reinterpret_cast<uint2 *>(counts)[i] = make_uint2(t.nbox, t.ntri);
 }
 
