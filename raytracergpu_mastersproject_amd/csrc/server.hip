@@ -171,6 +171,7 @@ struct rt_server {
     double last_small_ms = -1e30;
     double last_big_ms = -1e30;  // host time of the last big-lane step enqueued
     double slice_div = 8.0;      // RT_SERVER_NO_YIELD: big-lane slices while small requests arrive, budget / slice_div
+    double small_div = 5.0;      // a fused small-lane step's share of the budget: budget / small_div
     rt_server_stats stats{};
     rtgpu::Clock::time_point t0;
     std::thread worker;
@@ -326,8 +327,8 @@ bool enqueue_step(rt_server *s, bool big_lane) {
     // the split layout with one group size for the launch, chosen so the launch has ~32 work
     // items per resident lane: its drain (lanes finishing their last items) stays short and each
     // item traces its first segment once for its group of samples.  Such a step finishes all its
-    // members together, so it gets a fifth of the budget: its first members are not held long by
-    // its last.  Ray queries ride along (slices of every admitted query).
+    // members together, so it gets a fifth of the budget (RTGPU_SERVER_SMALL_DIV): its first
+    // members are not held long by its last.  Ray queries ride along (slices of every admitted query).
     //
     // Big lane: the oldest device-filling render (at least kWaveMinSamples pixel-samples at
     // admission, rt_render's own threshold) runs alone, bounce by bounce as rt_render would run
@@ -347,7 +348,7 @@ bool enqueue_step(rt_server *s, bool big_lane) {
     const bool big_step = !serial && big_lane && !renders.empty();
     const bool yields = !serial && !(s->cfg.flags & RT_SERVER_NO_YIELD);
     if (big_step) renders.resize(1);
-    else if (!serial && !renders.empty()) budget = std::max(1.0, budget / 5.0);
+    else if (!serial && !renders.empty()) budget = std::max(1.0, budget / s->small_div);
     if (big_step && small_recent && !yields) budget = std::max(1.0, budget / s->slice_div);
     for (auto &r : renders) {
         const uint64_t npix = npix_of(*r);
@@ -741,6 +742,7 @@ int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **o
         RT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         RT_HIP(hipStreamCreateWithFlags(&s->small_stream, hipStreamNonBlocking));
         if (const char *v = std::getenv("RTGPU_SERVER_SLICE_DIV")) s->slice_div = std::max(1.0, std::atof(v));
+        if (const char *v = std::getenv("RTGPU_SERVER_SMALL_DIV")) s->small_div = std::max(1.0, std::atof(v));
         if (const char *v = std::getenv("RTGPU_SERVER_YIELD_MIN_MS")) s->yield_min_ms = std::max(0.0, std::atof(v));
         RT_HIP(hipEventCreate(&s->t0_event));
         RT_HIP(hipEventRecord(s->t0_event, s->stream));
