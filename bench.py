@@ -408,11 +408,12 @@ def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int
     single-scene kernel; its saturated rate is the capacity against which the Poisson loads are set."""
     t0 = time.time()
     sizes = W.serving_scene_sizes()
-    sids, params = [], []
+    sids, params, bounds = [], [], []
     for k, nt in enumerate(sizes):
         s = W.served_scene(int(nt), seed=600 + k)
         sids.append(ctx.upload(s["tris"], None, s["materials"], width=build.width, leaf_size=build.leaf_size))
         params.append(s["params"])
+        bounds.append(W.scene_bounds(s["tris"]))
     resident = sum(ctx.info(s).device_bytes for s in sids)
     log(f"serving: {len(sids)} scenes ({int(sizes.sum())} triangles, {resident / 2**30:.2f} GiB resident) in "
         f"{time.time() - t0:.1f} s")
@@ -486,6 +487,7 @@ def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int
             f"{b['yields']} yields); without yielding small p99 {y['small']['p99_ms']}, device-filling p99 "
             f"{y['device_filling']['p99_ms']}; unbatched p50 {u['p50_ms']} p99 {u['p99_ms']} ms "
             f"(small p99 {u['small']['p99_ms']})")
+    queries = query_serving_bench(ctx, N, W, L, sids, bounds, 1000, arena, step_us)
     for s in sids:
         ctx.release(s)
     return {"workload": f"{len(sids)} resident scenes ({int(sizes.sum())} triangles, log-uniform 1e4..4194304), "
@@ -499,7 +501,59 @@ def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int
             # segments do not: the unbatched path keeps every primary hit in registers for all samples)
             "throughput_ratio_saturated": round(batched_sat["msamples_per_s"] / max(serial_sat["msamples_per_s"], 1e-9),
                                                 3),
-            "poisson": loads}
+            "poisson": loads, "queries": queries}
+
+
+def query_serving_bench(ctx, N, W, L, sids, bounds, nreq: int, arena: int, step_us: int):
+    """Ray-query requests through the same server and resident scenes: nreq queries (scene U{0..63}, batch
+    log-uniform 4,096..262,144 incoherent rays in the scene's box, closest hit, refined), their rays and hits
+    in device memory (RT_REQ_DEVICE_BUFFERS).  The unbatched server launches the plain single-scene kernel
+    once per request; the batched server fuses the admitted queries' slices into one launch per step.
+    Saturated rates, then p50/p99 at Poisson loads of 0.5x and 0.8x the unbatched capacity."""
+    rng = np.random.default_rng(11)
+    scene = rng.integers(0, len(sids), nreq)
+    sizes = np.exp(rng.uniform(np.log(4096), np.log(262144), nreq)).astype(np.int64)
+    t0 = time.time()
+    bufs = []
+    for k in range(nreq):
+        rays = W.incoherent_rays(int(sizes[k]), *bounds[scene[k]], seed=5000 + k)
+        bufs.append((N.DeviceArray.from_numpy(rays), N.DeviceArray(16 * len(rays))))
+    total = int(sizes.sum())
+    log(f"query serving: {nreq} queries, {total} rays (generated and resident in {time.time() - t0:.1f} s)")
+
+    def requests():
+        return [N.trace_request_device(sids[scene[k]], bufs[k][0], int(sizes[k]), bufs[k][1]) for k in range(nreq)]
+
+    def pct(a, q):
+        return round(float(np.percentile(a, q)), 3)
+
+    def run(serial: bool, arrivals):
+        srv = N.Server(ctx, arena_bytes=arena, max_active=64, step_us=step_us, serial=serial)
+        srv.replay(requests()[:4], np.zeros(4))
+        st = srv.replay(requests(), arrivals)
+        srv.close()
+        lat = np.array([x.done_ms - x.submit_ms for x in st])
+        makespan = max(x.done_ms for x in st) - float(arrivals[0])
+        return {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "requests_per_s": round(nreq / (makespan / 1e3), 1),
+                "mrays_per_s": round(total / (makespan / 1e3) / 1e6, 1)}
+
+    zero = np.zeros(nreq)
+    unb = run(True, zero)
+    bat = run(False, zero)
+    out = {"workload": f"{nreq} ray queries over the {len(sids)} resident scenes, batches log-uniform 4096..262144 "
+                       f"({total} rays), rays and hits in device memory; closest hit, t/u/v refined",
+           "saturated": {"batched": bat, "unbatched": unb},
+           "throughput_ratio_saturated": round(bat["mrays_per_s"] / max(unb["mrays_per_s"], 1e-9), 3),
+           "poisson": {}}
+    log(f"query serving: unbatched {unb['requests_per_s']} req/s ({unb['mrays_per_s']} Mrays/s), batched "
+        f"{bat['requests_per_s']} req/s ({bat['mrays_per_s']} Mrays/s), x{out['throughput_ratio_saturated']}")
+    for frac in (0.5, 0.8):
+        arr = W.poisson_arrivals(nreq, frac * unb["requests_per_s"], seed=12)
+        b, u = run(False, arr), run(True, arr)
+        out["poisson"][str(frac)] = {"qps": round(frac * unb["requests_per_s"], 1), "batched": b, "unbatched": u}
+        log(f"query serving at {frac}x ({frac * unb['requests_per_s']:.0f} req/s): batched p50 {b['p50_ms']} p99 "
+            f"{b['p99_ms']} ms; unbatched p50 {u['p50_ms']} p99 {u['p99_ms']} ms")
+    return out
 
 
 def main():

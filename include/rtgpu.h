@@ -319,6 +319,8 @@ typedef struct rt_server rt_server;
 #define RT_REQ_RENDER 0u
 #define RT_REQ_TRACE 1u
 #define RT_REQ_OUT_ACCUM 0x100u /* render_flags: return the raw accumulation instead of the resolved image */
+#define RT_REQ_DEVICE_BUFFERS 0x200u /* trace_flags: rays and out are device memory; the steps read the rays and
+                                       write the hits in place, with no copies (out is required) */
 
 #define RT_SERVER_SERIAL 1u /* one request per step, whole, through the single-scene kernels (the unbatched baseline) */
 #define RT_SERVER_NO_YIELD 2u /* device-filling renders run their slices to the end while small requests wait
@@ -346,10 +348,11 @@ typedef struct rt_request {
     rt_params params;
     /* ray query */
     uint32_t nrays;
-    uint32_t trace_flags;  /* RT_TRACE_ANY */
-    const rt_ray *rays;    /* host memory, copied at admission */
+    uint32_t trace_flags;  /* RT_TRACE_ANY | RT_TRACE_FP32 | RT_REQ_DEVICE_BUFFERS */
+    const rt_ray *rays;    /* host memory, copied at admission (device memory with RT_REQ_DEVICE_BUFFERS) */
     /* host memory receiving the result when the request completes (NULL = discard):
-     * render: width*height RGBA32F; ray query: nrays rt_hit */
+     * render: width*height RGBA32F; ray query: nrays rt_hit (device memory, written by the
+     * query's steps, with RT_REQ_DEVICE_BUFFERS) */
     void *out;
 } rt_request;
 

@@ -193,7 +193,8 @@ bool big_render(const rt_request &r, bool serial) {
 }
 
 uint64_t request_bytes(const rt_request &r, bool serial) {
-    if (r.kind != RT_REQ_RENDER) return Arena::round(32ull * r.nrays) + Arena::round(16ull * r.nrays);
+    if (r.kind != RT_REQ_RENDER)  // a query on device buffers keeps no copies in the arena
+        return (r.trace_flags & RT_REQ_DEVICE_BUFFERS) ? 0 : Arena::round(32ull * r.nrays) + Arena::round(16ull * r.nrays);
     const uint64_t image = Arena::round(16ull * r.width * r.height);
     return big_render(r, serial) ? 2 * image : image;
 }
@@ -292,7 +293,7 @@ void admit(rt_server *s) {
             hipStream_t st = lane_stream(s, r->big_lane);
             if (r->req.kind == RT_REQ_RENDER) {
                 launch_accum_clear(reinterpret_cast<float *>(arena_ptr(s, off)), r->req.width * r->req.height, st);
-            } else if (r->req.nrays) {
+            } else if (r->req.nrays && !(r->req.trace_flags & RT_REQ_DEVICE_BUFFERS)) {
                 RT_HIP(hipMemcpyAsync(arena_ptr(s, off), r->req.rays, 32ull * r->req.nrays, hipMemcpyHostToDevice, st));
             }
         }
@@ -463,8 +464,11 @@ bool enqueue_step(rt_server *s, bool big_lane) {
                                 : (uint32_t)std::min<double>(r->remaining, std::max(65536.0, std::floor(rshare)));
             q = std::max(1u, std::min(q, r->remaining));
             const uint32_t first = r->done_units;
-            const rt_ray *rays = reinterpret_cast<const rt_ray *>(arena_ptr(s, r->offset)) + first;
-            rt_hit *hits = reinterpret_cast<rt_hit *>(arena_ptr(s, r->offset + Arena::round(32ull * r->req.nrays))) + first;
+            const bool dev = (r->req.trace_flags & RT_REQ_DEVICE_BUFFERS) != 0;
+            const rt_ray *rays = (dev ? r->req.rays : reinterpret_cast<const rt_ray *>(arena_ptr(s, r->offset))) + first;
+            rt_hit *hits = (dev ? static_cast<rt_hit *>(r->req.out)
+                                : reinterpret_cast<rt_hit *>(arena_ptr(s, r->offset + Arena::round(32ull * r->req.nrays)))) +
+                           first;
             groups[r->req.trace_flags & kTraceFlags].push_back(TraceItem{r->scene, rays, hits, q});
             r->remaining -= q;
             r->done_units += q;
@@ -574,7 +578,7 @@ void retire_step(rt_server *s, size_t i, double done_ms) {
     s->stats.samples += step.samples;
     s->stats.rays += step.rays;
     for (auto &r : step.finishing) {
-        if (r->req.out) {
+        if (r->req.out && !(r->req.kind == RT_REQ_TRACE && (r->req.trace_flags & RT_REQ_DEVICE_BUFFERS))) {
             const uint64_t off = r->req.kind == RT_REQ_RENDER ? r->offset : r->offset + Arena::round(32ull * r->req.nrays);
             const uint64_t n = r->req.kind == RT_REQ_RENDER ? 16ull * r->req.width * r->req.height : 16ull * r->req.nrays;
             if (n) RT_HIP(hipMemcpy(r->req.out, arena_ptr(s, off), n, hipMemcpyDeviceToHost));
@@ -796,8 +800,10 @@ int rt_server_submit(rt_server *s, const rt_request *req, uint64_t *id) {
             RT_REQUIRE(req->spp > 0, "a render request needs at least one sample per pixel");
             r->remaining = req->spp;
         } else {
-            RT_REQUIRE((req->trace_flags & ~kTraceFlags) == 0, "unknown trace flags");
+            RT_REQUIRE((req->trace_flags & ~(kTraceFlags | RT_REQ_DEVICE_BUFFERS)) == 0, "unknown trace flags");
             RT_REQUIRE(req->nrays == 0 || req->rays, "rays is NULL");
+            RT_REQUIRE(!(req->trace_flags & RT_REQ_DEVICE_BUFFERS) || req->nrays == 0 || req->out,
+                       "a query on device buffers needs out");
             r->remaining = req->nrays;
         }
         {

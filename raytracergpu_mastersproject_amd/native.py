@@ -82,6 +82,7 @@ class RenderJob(C.Structure):
 REQ_RENDER = 0
 REQ_TRACE = 1
 REQ_OUT_ACCUM = 0x100
+REQ_DEVICE_BUFFERS = 0x200
 SERVER_SERIAL = 1
 SERVER_NO_YIELD = 2
 
@@ -466,6 +467,20 @@ def trace_request(scene_id, rays: np.ndarray, any_hit=False, out: np.ndarray | N
     if out is not None:
         assert out.flags.c_contiguous and out.nbytes >= 16 * len(rays)
         r.out = out.ctypes.data
+    return r
+
+
+def trace_request_device(scene_id, rays: "DeviceArray", n: int, hits: "DeviceArray", any_hit=False,
+                         fp32=False) -> Request:
+    """A ray query whose rays and hits stay in device memory (RT_REQ_DEVICE_BUFFERS): its steps read
+    the n rays in place and write the hits into `hits`, which holds them when the request completes."""
+    assert rays.nbytes >= 32 * n and hits.nbytes >= 16 * n
+    r = Request()
+    r.kind, r.scene_id, r.nrays = REQ_TRACE, scene_id, n
+    r.trace_flags = _trace_flags(any_hit, fp32) | REQ_DEVICE_BUFFERS
+    r.rays = rays.ptr if n else None
+    r.out = hits.ptr
+    r._keep = (rays, hits)
     return r
 
 

@@ -395,3 +395,41 @@ def test_device_filling_render_yields_to_small_requests(ctx, native, scenes, mon
     assert max(small) < lat[1]
     _check_results(native, ctx, ids, expect)
     ctx.release(ids["big"])
+
+
+@pytest.mark.parametrize("serial", [False, True])
+def test_queries_on_device_buffers(ctx, native, scenes, serial):
+    """Ray queries whose rays and hits stay in device memory (RT_REQ_DEVICE_BUFFERS), of 1 to 300,000
+    rays (the largest are cut into slices over several steps), closest and any hit, refined and fp32,
+    among host-buffer queries and a render: every query's hits, written in place by its steps, equal
+    rt_trace on the same rays alone, bit for bit.  They take no arena space, so a 4 MiB arena admits
+    them all beside the host-buffer query."""
+    ids, params, soup, ico = scenes
+    reqs, checks = [], []
+    sizes = [1, 4096, 65537, 300_000, 777]
+    for k, n in enumerate(sizes):
+        name = ("soup4", "ico", "soup8", "ico2", "soup4")[k]
+        tris = (soup if name.startswith("soup") else ico)["tris"]
+        rays = W.incoherent_rays(n, *W.scene_bounds(tris), seed=300 + k)
+        d_rays, d_hits = native.DeviceArray.from_numpy(rays), native.DeviceArray(16 * n)
+        any_hit, fp32 = k % 2 == 1, k == 2
+        reqs.append(native.trace_request_device(ids[name], d_rays, n, d_hits, any_hit=any_hit, fp32=fp32))
+        checks.append((name, rays, any_hit, fp32, d_hits))
+    host_rays, _ = _rays(ids, (soup, ico))["soup4"]
+    host_out = np.zeros(len(host_rays), L.HIT)
+    reqs.insert(2, native.trace_request(ids["soup4"], host_rays, out=host_out))
+    img = np.zeros((32 * 24, 4), np.float32)
+    p = with_state(params["ico"], 9)
+    reqs.insert(4, native.render_request(ids["ico"], p, 32, 24, 3, native.REQ_OUT_ACCUM, out=img))
+    arena = (4 << 20) if not serial else (64 << 20)
+    srv = native.Server(ctx, arena_bytes=arena, max_active=8, step_us=2000, serial=serial)
+    stats = srv.replay(reqs, np.arange(len(reqs), dtype=np.float64) * 0.2)
+    srv.close()
+    assert all(s.status == 0 for s in stats)
+    for name, rays, any_hit, fp32, d_hits in checks:
+        got = d_hits.to_numpy(L.HIT, len(rays))
+        ref = gpu_trace(ctx, ids[name], rays, any_hit=any_hit, fp32=fp32)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (name, len(rays), any_hit, fp32)
+    ref = gpu_trace(ctx, ids["soup4"], host_rays)
+    assert np.array_equal(host_out.view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(img.view(np.uint32), render_alone(native, ctx, ids["ico"], p, 32, 24, 3).view(np.uint32))
