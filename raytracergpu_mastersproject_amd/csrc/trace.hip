@@ -423,7 +423,10 @@ void launch_trace(rt_context *ctx, const Scene &sc, const rt_ray *rays, uint32_t
         a.work = ctx->counter_slot();
         k_zero_counter<<<1, 64, 0, stream>>>(a.work);
         void *k = pick_kernel(sc.width, c.lds_stack, refine_in_kernel ? 3 : 1, any, counts != nullptr);
-        uint32_t grid = persistent_grid(ctx, k, c.block, lds);
+        // no more lanes than rays: a small batch would otherwise launch the whole device's worth of
+        // waves, nearly all of which find no chunk and exit
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(persistent_grid(ctx, k, c.block, lds),
+                                                           ((uint64_t)n + c.block - 1) / c.block);
         void *args[] = {&a};
         RT_HIP(hipLaunchKernel(k, dim3(grid), dim3(c.block), args, lds, stream));
     }
