@@ -540,13 +540,28 @@ def query_serving_bench(ctx, N, W, L, sids, bounds, nreq: int, arena: int, step_
     zero = np.zeros(nreq)
     unb = run(True, zero)
     bat = run(False, zero)
+    # no server: every query's rt_trace back to back on one stream (no arrivals, no latency)
+    stream = N.Stream()
+    for k in range(4):
+        ctx.trace(sids[scene[k]], bufs[k][0], int(sizes[k]), bufs[k][1], stream=stream)
+    ev = [N.Event(), N.Event()]
+    ev[0].record(stream)
+    for k in range(nreq):
+        ctx.trace(sids[scene[k]], bufs[k][0], int(sizes[k]), bufs[k][1], stream=stream)
+    ev[1].record(stream)
+    stream.synchronize()
+    loop_ms = ev[0].elapsed_ms(ev[1])
     out = {"workload": f"{nreq} ray queries over the {len(sids)} resident scenes, batches log-uniform 4096..262144 "
                        f"({total} rays), rays and hits in device memory; closest hit, t/u/v refined",
            "saturated": {"batched": bat, "unbatched": unb},
            "throughput_ratio_saturated": round(bat["mrays_per_s"] / max(unb["mrays_per_s"], 1e-9), 3),
+           "plain_loop": {"ms": round(loop_ms, 2), "mrays_per_s": round(total / (loop_ms * 1e-3) / 1e6, 1)},
            "poisson": {}}
+    out["throughput_ratio_vs_plain_loop"] = round(bat["mrays_per_s"] / out["plain_loop"]["mrays_per_s"], 3)
     log(f"query serving: unbatched {unb['requests_per_s']} req/s ({unb['mrays_per_s']} Mrays/s), batched "
-        f"{bat['requests_per_s']} req/s ({bat['mrays_per_s']} Mrays/s), x{out['throughput_ratio_saturated']}")
+        f"{bat['requests_per_s']} req/s ({bat['mrays_per_s']} Mrays/s), x{out['throughput_ratio_saturated']}; "
+        f"plain rt_trace loop without a server {out['plain_loop']['mrays_per_s']} Mrays/s "
+        f"(batched x{out['throughput_ratio_vs_plain_loop']})")
     for frac in (0.5, 0.8):
         arr = W.poisson_arrivals(nreq, frac * unb["requests_per_s"], seed=12)
         b, u = run(False, arr), run(True, arr)
