@@ -116,9 +116,10 @@ def test_fused_many_scene_query_at_full_size(ctx, native, O):
 def test_serving_at_full_size(ctx, native):
     """The serving workload itself: 64 resident scenes (triangle counts log-uniform 1e4..4,194,304) and the
     benchmark's 160 render requests (256^2, 512^2 or 1920x1080, 1..64 spp, depth 8, seed = request id),
-    plus 16 ray-query requests of 2^20 rays, replayed at Poisson arrivals through the continuous-batching
-    server with a 16 GiB arena.  Every served result equals the request run alone (rt_render or
-    rt_trace), bit for bit, including the device-filling renders whose passes yielded to small requests."""
+    plus 16 ray-query requests of 2^20 rays (4 of them on device buffers), replayed at Poisson arrivals
+    through the continuous-batching server with a 16 GiB arena.  Every served result equals the request
+    run alone (rt_render or rt_trace), bit for bit, including the device-filling renders whose passes
+    yielded to small requests."""
     sizes = W.serving_scene_sizes()
     sids, params, bounds = [], [], []
     for k, nt in enumerate(sizes):
@@ -139,6 +140,12 @@ def test_serving_at_full_size(ctx, native):
     for k in range(16):  # ray queries on every fourth scene, closest and any hit, among the renders
         sc = 4 * k
         rays = W.incoherent_rays(1 << 20, *bounds[sc], seed=900 + k)
+        if k % 4 == 3:  # rays and hits in device memory
+            d_rays, d_hits = native.DeviceArray.from_numpy(rays), native.DeviceArray(16 * len(rays))
+            reqs.insert(10 * k + 5, native.trace_request_device(sids[sc], d_rays, len(rays), d_hits,
+                                                                any_hit=k % 2 == 1))
+            outs.insert(10 * k + 5, ("trace", sids[sc], rays, k % 2 == 1, d_hits))
+            continue
         out = np.zeros(len(rays), L.HIT)
         reqs.insert(10 * k + 5, native.trace_request(sids[sc], rays, any_hit=k % 2 == 1, out=out))
         outs.insert(10 * k + 5, ("trace", sids[sc], rays, k % 2 == 1, out))
@@ -162,6 +169,8 @@ def test_serving_at_full_size(ctx, native):
             assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), d
         else:
             _, sid, rays, any_hit, out = o
+            if isinstance(out, native.DeviceArray):
+                out = out.to_numpy(L.HIT, len(rays))
             dr, dh = native.DeviceArray.from_numpy(rays), native.DeviceArray(16 * len(rays))
             ctx.trace(sid, dr, len(rays), dh, any_hit=any_hit)
             assert np.array_equal(out.view(np.uint32), dh.to_numpy(L.HIT, len(rays)).view(np.uint32))
