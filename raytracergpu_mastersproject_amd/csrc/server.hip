@@ -481,8 +481,10 @@ bool enqueue_step(rt_server *s, bool big_lane) {
             if (items.empty()) continue;
             LaunchCfg tc = resolve_cfg(s->ctx, *items[0].sc, &s->cfg.trace_cfg, (flags & RT_TRACE_ANY) != 0);
             if (!s->cfg.trace_cfg.persistent) tc.persistent = 1;
-            if (serial) launch_trace(s->ctx, *items[0].sc, items[0].rays, items[0].n, items[0].hits, nullptr, flags, tc,
-                                     stream);
+            // a lone query goes through the single-scene kernel: the fused launch's item table and
+            // per-chunk item lookup only pay off over several
+            if (serial || items.size() == 1)
+                launch_trace(s->ctx, *items[0].sc, items[0].rays, items[0].n, items[0].hits, nullptr, flags, tc, stream);
             else launch_trace_items(s->ctx, items, flags, tc, stream);
         }
     }
