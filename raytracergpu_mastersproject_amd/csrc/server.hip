@@ -175,6 +175,12 @@ struct rt_server {
     rt_server_stats stats{};
     rtgpu::Clock::time_point t0;
     std::thread worker;
+    // Results bound for the callers' host buffers are copied on a thread of their own (completer_main),
+    // so the worker keeps scheduling while a large image crosses the host link.
+    std::deque<std::shared_ptr<rtgpu::Request>> completing;
+    std::condition_variable cv_complete;
+    bool completer_stop = false;
+    std::thread completer;
 
     double now_ms() const {
         return std::chrono::duration<double, std::milli>(rtgpu::Clock::now() - t0).count();
@@ -557,6 +563,29 @@ void resume_step(rt_server *s, Step &from) {
     s->inflight.push_back(std::move(rs));
 }
 
+// A request whose result is where the caller wants it: its scene pin and arena range go back.  Called
+// with s->m held.
+void complete_request(rt_server *s, const std::shared_ptr<Request> &r, double done_ms) {
+    if (r->state == State::Done) return;  // failed meanwhile (fail_all)
+    {
+        std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+        r->scene->users--;
+    }
+    s->arena.release(r->offset, r->bytes);
+    r->state = State::Done;
+    r->st.done_ms = done_ms;
+    r->st.status = RT_OK;
+    s->stats.completed++;
+    auto it = std::find(s->active.begin(), s->active.end(), r);
+    if (it != s->active.end()) s->active.erase(it);
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();  // the freed range may admit the queue head; the worker may be waiting to stop
+}
+
+bool copies_out(const Request &r) {
+    return r.req.out && !(r.req.kind == RT_REQ_TRACE && (r.req.trace_flags & RT_REQ_DEVICE_BUFFERS));
+}
+
 // Completes inflight[i] (its end event has fired).  Called with s->m held.  A lane's steps retire
 // in order, so a request's last step retires after its others.
 void retire_step(rt_server *s, size_t i, double done_ms) {
@@ -580,26 +609,45 @@ void retire_step(rt_server *s, size_t i, double done_ms) {
     s->stats.samples += step.samples;
     s->stats.rays += step.rays;
     for (auto &r : step.finishing) {
-        if (r->req.out && !(r->req.kind == RT_REQ_TRACE && (r->req.trace_flags & RT_REQ_DEVICE_BUFFERS))) {
-            const uint64_t off = r->req.kind == RT_REQ_RENDER ? r->offset : r->offset + Arena::round(32ull * r->req.nrays);
-            const uint64_t n = r->req.kind == RT_REQ_RENDER ? 16ull * r->req.width * r->req.height : 16ull * r->req.nrays;
-            if (n) RT_HIP(hipMemcpy(r->req.out, arena_ptr(s, off), n, hipMemcpyDeviceToHost));
+        if (copies_out(*r)) {  // completed by completer_main once the result is in the caller's buffer
+            s->completing.push_back(r);
+            s->cv_complete.notify_one();
+        } else {
+            complete_request(s, r, done_ms);
         }
-        {
-            std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
-            r->scene->users--;
-        }
-        s->arena.release(r->offset, r->bytes);
-        r->state = State::Done;
-        r->st.done_ms = done_ms;
-        r->st.status = RT_OK;
-        s->stats.completed++;
-        s->active.erase(std::find(s->active.begin(), s->active.end(), r));
     }
     if (step.scratch_bytes) s->scratch.release(step.scratch_off, step.scratch_bytes);
     s->event_pool.push_back(step.start);
     s->event_pool.push_back(step.end);
     s->cv_done.notify_all();
+}
+
+void fail_all(rt_server *s, int status, const std::string &msg);
+
+// Copies finished requests' results to the callers' host buffers, oldest first, then completes them.
+void completer_main(rt_server *s) {
+    std::unique_lock<std::mutex> lk(s->m);
+    hipError_t err = hipSetDevice(s->ctx->device);
+    for (;;) {
+        s->cv_complete.wait(lk, [&] { return !s->completing.empty() || s->completer_stop; });
+        if (s->completing.empty()) break;  // asked to stop, nothing left
+        std::shared_ptr<Request> r = s->completing.front();
+        s->completing.pop_front();
+        const bool render = r->req.kind == RT_REQ_RENDER;
+        const uint64_t off = render ? r->offset : r->offset + Arena::round(32ull * r->req.nrays);
+        const uint64_t n = render ? 16ull * r->req.width * r->req.height : 16ull * r->req.nrays;
+        const void *src = arena_ptr(s, off);  // the range stays allocated until complete_request
+        lk.unlock();
+        if (err == hipSuccess && n) err = hipMemcpy(r->req.out, src, n, hipMemcpyDeviceToHost);
+        lk.lock();
+        if (err != hipSuccess) {
+            fail_all(s, RT_ERR_HIP, std::string("copying a result to the host: ") + hipGetErrorString(err));
+            s->completing.clear();
+            s->cv_work.notify_all();
+            continue;
+        }
+        complete_request(s, r, s->now_ms());
+    }
 }
 
 void fail_all(rt_server *s, int status, const std::string &msg) {
@@ -755,6 +803,7 @@ int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **o
         s->stats.arena_bytes = s->arena.size();
         s->t0 = Clock::now();
         s->worker = std::thread(worker_main, s.get());
+        s->completer = std::thread(completer_main, s.get());
     } catch (const Error &e) {
         ctx->last_error = e.what();
         if (s) destroy_streams(s.get());
@@ -775,7 +824,13 @@ int rt_server_destroy(rt_server *s) {
         s->stop = true;
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    if (s->worker.joinable()) s->worker.join();  // returns once every request has completed or failed
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->completer_stop = true;
+    }
+    s->cv_complete.notify_all();
+    if (s->completer.joinable()) s->completer.join();
     const int st = s->fatal;
     for (hipStream_t st : {s->stream, s->small_stream})
         if (st) (void)hipStreamSynchronize(st);
