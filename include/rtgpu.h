@@ -157,8 +157,8 @@ int rt_scene_get_wide(rt_context *ctx, uint32_t scene_id, void *host_out, size_t
 #define RT_TRACE_FP32 2u
 
 typedef struct rt_launch_config {
-    uint32_t block;        /* threads per block: 64, 128, 256 (0 = tuned default)       */
-    uint32_t lds_stack;    /* LDS short-stack entries per lane: 4, 8, 16 (0 = default)  */
+    uint32_t block;        /* threads per block: 64, 128, 256 (0 = tuned, else 128)     */
+    uint32_t lds_stack;    /* LDS short-stack entries per lane: 4, 8, 16 (0 = 16)       */
     uint32_t persistent;   /* 1 = persistent work-stealing grid, 0 = one lane per ray    */
     uint32_t refill;       /* persistent: refill a wave when >= this many lanes are idle */
 } rt_launch_config;

@@ -68,7 +68,7 @@ struct DevBuf {
 // Launch configuration actually used by a kernel (after defaults are applied).
 struct LaunchCfg {
     uint32_t block = 128;
-    uint32_t lds_stack = 8;
+    uint32_t lds_stack = 16;
     uint32_t persistent = 1;
     uint32_t refill = 16;
     uint32_t leaf_weight = 2;  // leaf phase when leaf lanes * weight >= node lanes
