@@ -280,10 +280,15 @@ size_t lane_inflight(const rt_server *s, bool big_lane) {
     return n;
 }
 
-// Admits queue heads while a slot and arena space are free.  Called with s->m held.
+// Admits queue heads while a slot and arena space are free.  Called with s->m held.  A slot is held
+// by a request with work not yet enqueued; one whose last slice is on the device holds only its arena
+// range, so the next step can be formed while that one runs (otherwise a step that takes every
+// admitted query whole leaves the lane idle from its end until the host has seen it finish).
 void admit(rt_server *s) {
     const uint32_t slots = (s->cfg.flags & RT_SERVER_SERIAL) ? std::max(1u, s->cfg.pipeline) : s->cfg.max_active;
-    while (!s->queue.empty() && s->active.size() < slots) {
+    size_t busy = 0;
+    for (const auto &r : s->active) busy += (r->state == State::Active && r->remaining > 0) ? 1 : 0;
+    while (!s->queue.empty() && busy < slots) {
         std::shared_ptr<Request> r = s->queue.front();
         uint64_t off = 0;
         if (!s->arena.alloc(r->bytes, off)) break;  // arrival order: the head waits for memory
@@ -304,6 +309,7 @@ void admit(rt_server *s) {
             }
         }
         s->active.push_back(r);
+        busy += r->remaining > 0 ? 1 : 0;
     }
 }
 
