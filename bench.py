@@ -401,7 +401,8 @@ def batched_bench(ctx, N, W, L, nscenes: int, reps: int):
     return out, best
 
 
-def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int, max_spp_step: int = 0):
+def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int, max_spp_step: int = 0,
+                  max_active: int = 256):
     """The serving loop: 64 resident scenes (triangle counts log-uniform 1e4..4,194,304), render requests
     (scene U{0..63}, resolution U{256^2, 512^2, 1920x1080}, spp U{1..64}, depth 8, seed = request id)
     admitted by an arena budget.  The unbatched baseline serves one request at a time with the plain
@@ -435,7 +436,7 @@ def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int
         return round(float(np.percentile(a, q)), 2) if len(a) else None
 
     def run(serial: bool, arrivals, reqs=None, sel=None, no_yield=False):
-        srv = N.Server(ctx, arena_bytes=arena, max_active=64, step_us=step_us, serial=serial,
+        srv = N.Server(ctx, arena_bytes=arena, max_active=max_active, step_us=step_us, serial=serial,
                        max_spp_step=max_spp_step, no_yield=no_yield)
         warm = requests()[:4]
         srv.replay(warm, np.zeros(len(warm)))
@@ -487,13 +488,13 @@ def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int
             f"{b['yields']} yields); without yielding small p99 {y['small']['p99_ms']}, device-filling p99 "
             f"{y['device_filling']['p99_ms']}; unbatched p50 {u['p50_ms']} p99 {u['p99_ms']} ms "
             f"(small p99 {u['small']['p99_ms']})")
-    queries = query_serving_bench(ctx, N, W, L, sids, bounds, 1000, arena, step_us)
+    queries = query_serving_bench(ctx, N, W, L, sids, bounds, 1000, arena, step_us, max_active)
     for s in sids:
         ctx.release(s)
     return {"workload": f"{len(sids)} resident scenes ({int(sizes.sum())} triangles, log-uniform 1e4..4194304), "
                         f"{nreq} render requests: resolution U{{256^2, 512^2, 1920x1080}}, spp U{{1..64}}, depth 8, "
                         f"seed = request id; Poisson arrivals",
-            "arena_bytes": arena, "step_us": step_us, "resident_bytes": int(resident),
+            "arena_bytes": arena, "step_us": step_us, "max_active": max_active, "resident_bytes": int(resident),
             "classes": "device_filling: width x height x spp >= 2^22 (rendered bounce by bounce); small: the rest",
             "alone": alone_cls,
             "unbatched_capacity_rps": cap, "saturated": {"batched": batched_sat, "unbatched": serial_sat},
@@ -504,7 +505,7 @@ def serving_bench(ctx, N, W, L, nreq: int, build, arena_gib: float, step_us: int
             "poisson": loads, "queries": queries}
 
 
-def query_serving_bench(ctx, N, W, L, sids, bounds, nreq: int, arena: int, step_us: int):
+def query_serving_bench(ctx, N, W, L, sids, bounds, nreq: int, arena: int, step_us: int, max_active: int = 256):
     """Ray-query requests through the same server and resident scenes: nreq queries (scene U{0..63}, batch
     log-uniform 4,096..262,144 incoherent rays in the scene's box, closest hit, refined), their rays and hits
     in device memory (RT_REQ_DEVICE_BUFFERS).  The unbatched server launches the plain single-scene kernel
@@ -528,7 +529,7 @@ def query_serving_bench(ctx, N, W, L, sids, bounds, nreq: int, arena: int, step_
         return round(float(np.percentile(a, q)), 3)
 
     def run(serial: bool, arrivals):
-        srv = N.Server(ctx, arena_bytes=arena, max_active=64, step_us=step_us, serial=serial)
+        srv = N.Server(ctx, arena_bytes=arena, max_active=max_active, step_us=step_us, serial=serial)
         srv.replay(requests()[:4], np.zeros(4))
         st = srv.replay(requests(), arrivals)
         srv.close()
@@ -596,6 +597,7 @@ def main():
     ap.add_argument("--serve-arena-gib", type=float, default=16.0)
     ap.add_argument("--serve-step-us", type=int, default=50000)
     ap.add_argument("--serve-max-spp-step", type=int, default=0)
+    ap.add_argument("--serve-max-active", type=int, default=256)
     ap.add_argument("--traffic-sweep", action="store_true",
                     help="under rocprofv3 --pmc: a copy calibration, then one traversal launch per tuner configuration")
     ap.add_argument("--traffic-table", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "SWEEP_LOG"),
@@ -796,7 +798,7 @@ def main():
         out["batched"], build = batched_bench(ctx, N, W, L, args.batched_scenes, 5)
     if not args.no_serving:
         out["serving"] = serving_bench(ctx, N, W, L, args.serve_requests, build, args.serve_arena_gib,
-                                       args.serve_step_us, args.serve_max_spp_step)
+                                       args.serve_step_us, args.serve_max_spp_step, args.serve_max_active)
     print(json.dumps(out), flush=True)
 
 

@@ -328,7 +328,7 @@ typedef struct rt_server rt_server;
 
 typedef struct rt_server_config {
     uint64_t arena_bytes;        /* device memory for request buffers: the admission budget (0 = 4 GiB) */
-    uint32_t max_active;         /* admitted requests with work not yet enqueued (0 = 64) */
+    uint32_t max_active;         /* admitted requests with work not yet enqueued (0 = 256) */
     uint32_t step_us;            /* GPU time a scheduling step may take at most (0 = 50000) */
     uint32_t pipeline;           /* steps enqueued ahead of the device (0 = 2) */
     uint32_t flags;              /* RT_SERVER_* */

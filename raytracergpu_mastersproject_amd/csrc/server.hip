@@ -781,7 +781,7 @@ int rt_server_create(rt_context *ctx, const rt_server_config *cfg, rt_server **o
         s->ctx = ctx;
         if (cfg) s->cfg = *cfg;
         if (!s->cfg.arena_bytes) s->cfg.arena_bytes = 4ull << 30;
-        if (!s->cfg.max_active) s->cfg.max_active = 64;
+        if (!s->cfg.max_active) s->cfg.max_active = 256;
         if (!s->cfg.step_us) s->cfg.step_us = 50000;
         if (!s->cfg.pipeline) s->cfg.pipeline = 2;
         s->cfg.pipeline = std::min(s->cfg.pipeline, 8u);

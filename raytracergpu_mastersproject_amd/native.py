@@ -487,7 +487,7 @@ def trace_request_device(scene_id, rays: "DeviceArray", n: int, hits: "DeviceArr
 class Server:
     """A serving loop over a context's resident scenes (rt_server_*)."""
 
-    def __init__(self, ctx: Context, arena_bytes=1 << 30, max_active=64, step_us=50000, pipeline=2, serial=False,
+    def __init__(self, ctx: Context, arena_bytes=1 << 30, max_active=256, step_us=50000, pipeline=2, serial=False,
                  max_spp_step=0, render_cfg: LaunchConfig | None = None, trace_cfg: LaunchConfig | None = None,
                  no_yield=False):
         flags = (SERVER_SERIAL if serial else 0) | (SERVER_NO_YIELD if no_yield else 0)
