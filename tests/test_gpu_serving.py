@@ -433,3 +433,27 @@ def test_queries_on_device_buffers(ctx, native, scenes, serial):
     ref = gpu_trace(ctx, ids["soup4"], host_rays)
     assert np.array_equal(host_out.view(np.uint32), ref.view(np.uint32))
     assert np.array_equal(img.view(np.uint32), render_alone(native, ctx, ids["ico"], p, 32, 24, 3).view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_next_query_is_admitted_while_the_last_step_runs(ctx, native, scenes):
+    """max_active bounds the admitted requests that still have work to enqueue: with one slot, a query
+    whose rays are all in an enqueued step no longer holds it, so the next query is admitted (and its
+    step enqueued) before the previous one completes.  Every query's hits equal rt_trace alone."""
+    ids, params, soup, ico = scenes
+    tris = ico["tris"]
+    reqs, checks = [], []
+    for k in range(12):
+        rays = W.incoherent_rays(20_000, *W.scene_bounds(tris), seed=700 + k)
+        d_rays, d_hits = native.DeviceArray.from_numpy(rays), native.DeviceArray(16 * len(rays))
+        reqs.append(native.trace_request_device(ids["ico"], d_rays, len(rays), d_hits))
+        checks.append((rays, d_hits))
+    srv = native.Server(ctx, arena_bytes=4 << 20, max_active=1, step_us=50000)
+    stats = srv.replay(reqs, np.zeros(len(reqs)))
+    srv.close()
+    assert all(s.status == 0 for s in stats)
+    for rays, d_hits in checks:
+        got = d_hits.to_numpy(L.HIT, len(rays))
+        assert np.array_equal(got.view(np.uint32), gpu_trace(ctx, ids["ico"], rays).view(np.uint32))
+    early = sum(stats[k + 1].admit_ms < stats[k].done_ms for k in range(len(stats) - 1))
+    assert early >= (len(stats) - 1) // 2, [(s.admit_ms, s.done_ms) for s in stats]
