@@ -372,7 +372,9 @@ typedef struct rt_server_stats {
     uint64_t arena_bytes, arena_peak;
     double busy_ms;        /* device time of all steps */
     double wall_ms;        /* since the server was created */
-    uint32_t active, queued;
+    uint32_t active;       /* admitted and not yet complete; may exceed max_active, since a request
+                            * whose last slice is enqueued no longer counts against it */
+    uint32_t queued;       /* submitted and not yet admitted */
     uint64_t yields;       /* passes of device-filling renders resumed after yielding to small requests */
 } rt_server_stats;
 
